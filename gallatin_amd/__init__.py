@@ -1,0 +1,205 @@
+"""MI355X-native Gallatin: a device-side malloc/free for gfx950.
+
+Host mirror of the reference's host interface (include/gallatin/allocators/
+gallatin.cuh:324-421, 643-668, 1855-1897 and global_allocator.cuh:48-97),
+backed by the HIP library gallatin_amd/lib/libgallatin_amd.so. The device
+interface (Gallatin<SEG,MIN,MAX>::malloc/free, global_malloc/global_free) is
+the C++ header set in gallatin_amd/include/gallatin/.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import (  # noqa: F401
+    GAL_NULL_OFFSET,
+    GAL_OP_FREE,
+    GAL_OP_MALLOC,
+    GAL_VERIFY_ATOMIC,
+    GAL_VERIFY_STORE,
+    GallatinError,
+    GalOp,
+)
+
+DEFAULT_SEGMENT = 16 << 20
+DEFAULT_MIN = 16
+DEFAULT_MAX = 4096
+
+_lib = _capi.lib
+
+
+def supported_triples():
+    buf = (ctypes.c_uint64 * 96)()
+    n = _lib.gal_supported(buf, 32)
+    return [tuple(buf[3 * i : 3 * i + 3]) for i in range(n)]
+
+
+def ops_array(ops):
+    """ops: iterable of (kind, arg, tid, slot) -> ctypes array of gal_op."""
+    ops = list(ops)
+    arr = (GalOp * max(1, len(ops)))()
+    for i, (kind, arg, tid, slot) in enumerate(ops):
+        arr[i].kind = kind
+        arr[i].arg = arg
+        arr[i].tid = tid
+        arr[i].slot = slot
+    return arr, len(ops)
+
+
+class Gallatin:
+    """Host handle of one device allocator Gallatin<segment_size, min_size, max_size>."""
+
+    def __init__(self, handle, info):
+        self._h = handle
+        self.info = info
+
+    # reference gallatin.cuh:324 generate_on_device(max_bytes, seed, print_info, running_calloc)
+    @classmethod
+    def generate_on_device(
+        cls,
+        max_bytes,
+        seed,
+        print_info=False,
+        segment_size=DEFAULT_SEGMENT,
+        min_size=DEFAULT_MIN,
+        max_size=DEFAULT_MAX,
+    ):
+        h = ctypes.c_void_p()
+        _capi.check("gal_create", _lib.gal_create(segment_size, min_size, max_size, max_bytes, seed, ctypes.byref(h)))
+        info = _capi.GalInfo()
+        _capi.check("gal_get_info", _lib.gal_get_info(h, ctypes.byref(info)))
+        obj = cls(h, info)
+        if print_info:
+            obj.print_info()
+        return obj
+
+    # reference gallatin.cuh:643 free_on_device
+    def free_on_device(self):
+        if self._h is not None:
+            _capi.check("gal_destroy", _lib.gal_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free_on_device()
+
+    def __del__(self):
+        try:
+            self.free_on_device()
+        except Exception:
+            pass
+
+    # reference gallatin.cuh:1855 print_info
+    def print_info(self):
+        _capi.check("gal_print_stats", _lib.gal_print_stats(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def errors(self):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _capi.check("gal_errors", _lib.gal_errors(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def snapshot(self):
+        need = ctypes.c_uint64()
+        _lib.gal_snapshot(self._h, None, 0, ctypes.byref(need))
+        buf = np.zeros(need.value, dtype=np.uint8)
+        _capi.check(
+            "gal_snapshot", _lib.gal_snapshot(self._h, buf.ctypes.data_as(ctypes.c_void_p), need.value, None)
+        )
+        return buf
+
+    def replay(self, ops):
+        arr, n = ops_array(ops)
+        out = (ctypes.c_uint64 * max(1, n))()
+        _capi.check("gal_replay", _lib.gal_replay(self._h, arr, n, out))
+        return np.frombuffer(out, dtype=np.uint64, count=n).copy()
+
+    def run_plain(self, n_threads, size, seed, verify=GAL_VERIFY_STORE):
+        r = _capi.GalRunResult()
+        _capi.check("gal_run_plain", _lib.gal_run_plain(self._h, n_threads, size, seed, verify, ctypes.byref(r)))
+        return r.as_dict()
+
+    def mixed_reset(self, threads_per_round, rounds, seed):
+        _capi.check("gal_mixed_reset", _lib.gal_mixed_reset(self._h, threads_per_round, rounds, seed))
+
+    def mixed_round(self, rnd, verify=GAL_VERIFY_STORE):
+        r = _capi.GalRunResult()
+        _capi.check("gal_mixed_round", _lib.gal_mixed_round(self._h, rnd, verify, ctypes.byref(r)))
+        return r.as_dict()
+
+    def mixed_drain(self, verify=GAL_VERIFY_STORE):
+        r = _capi.GalRunResult()
+        _capi.check("gal_mixed_drain", _lib.gal_mixed_drain(self._h, verify, ctypes.byref(r)))
+        return r.as_dict()
+
+    def mixed_live(self):
+        n = ctypes.c_uint64()
+        _capi.check("gal_mixed_live", _lib.gal_mixed_live(self._h, None, None, None, 0, ctypes.byref(n)))
+        cnt = n.value
+        offs = np.zeros(max(1, cnt), dtype=np.uint64)
+        sizes = np.zeros(max(1, cnt), dtype=np.uint64)
+        slots = np.zeros(max(1, cnt), dtype=np.uint64)
+        p = ctypes.POINTER(ctypes.c_uint64)
+        _capi.check(
+            "gal_mixed_live",
+            _lib.gal_mixed_live(
+                self._h,
+                offs.ctypes.data_as(p),
+                sizes.ctypes.data_as(p),
+                slots.ctypes.data_as(p),
+                cnt,
+                ctypes.byref(n),
+            ),
+        )
+        return offs[:cnt], sizes[:cnt], slots[:cnt]
+
+    def run_mixed(self, threads_per_round, rounds, seed, verify=GAL_VERIFY_STORE):
+        r = _capi.GalRunResult()
+        _capi.check(
+            "gal_run_mixed", _lib.gal_run_mixed(self._h, threads_per_round, rounds, seed, verify, ctypes.byref(r))
+        )
+        return r.as_dict()
+
+    def latency_ns(self):
+        """Per-wave malloc latencies (ns) of the last mixed run."""
+        n, tick = ctypes.c_uint64(), ctypes.c_double()
+        _capi.check("gal_latency_samples", _lib.gal_latency_samples(self._h, None, 0, ctypes.byref(n), ctypes.byref(tick)))
+        buf = np.zeros(max(1, n.value), dtype=np.uint32)
+        _capi.check(
+            "gal_latency_samples",
+            _lib.gal_latency_samples(
+                self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n.value, ctypes.byref(n), None
+            ),
+        )
+        return buf[: n.value].astype(np.float64) * tick.value
+
+    def read_pool(self, offset, nbytes):
+        buf = np.zeros(nbytes, dtype=np.uint8)
+        _capi.check(
+            "gal_read_pool", _lib.gal_read_pool(self._h, offset, buf.ctypes.data_as(ctypes.c_void_p), nbytes)
+        )
+        return buf
+
+
+# ---- global allocator (reference global_allocator.cuh:41-97)
+def init_global_allocator(num_bytes, seed, print_info=True):
+    _capi.check("gal_init_global_allocator", _lib.gal_init_global_allocator(num_bytes, seed, int(bool(print_info))))
+
+
+def free_global_allocator():
+    _capi.check("gal_free_global_allocator", _lib.gal_free_global_allocator())
+
+
+def print_global_stats():
+    _capi.check("gal_print_global_stats", _lib.gal_print_global_stats())
+
+
+def global_smoke(n_threads, size):
+    m, t = ctypes.c_uint64(), ctypes.c_uint64()
+    _capi.check("gal_global_smoke", _lib.gal_global_smoke(n_threads, size, ctypes.byref(m), ctypes.byref(t)))
+    return m.value, t.value

@@ -1,0 +1,134 @@
+"""ctypes binding of libgallatin_amd.so (the C-ABI in include/gallatin_amd.h).
+
+The library is the product: there is no CPU fallback. If it has not been
+built, importing this module raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgallatin_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gallatin_amd.h")
+
+GAL_OK = 0
+GAL_ERR_INVALID = 1
+GAL_ERR_UNSUPPORTED = 2
+GAL_ERR_OOM = 3
+GAL_ERR_HIP = 4
+GAL_ERR_BUFFER = 5
+GAL_ERR_STATE = 6
+ERROR_NAMES = {
+    GAL_OK: "ok",
+    GAL_ERR_INVALID: "invalid argument",
+    GAL_ERR_UNSUPPORTED: "unsupported (segment, min, max) triple",
+    GAL_ERR_OOM: "out of device memory",
+    GAL_ERR_HIP: "HIP runtime error",
+    GAL_ERR_BUFFER: "buffer too small",
+    GAL_ERR_STATE: "allocator state error",
+}
+
+GAL_OP_MALLOC = 0
+GAL_OP_FREE = 1
+GAL_NULL_OFFSET = (1 << 64) - 1
+GAL_VERIFY_STORE = 0
+GAL_VERIFY_ATOMIC = 1
+
+u64 = ctypes.c_uint64
+u32 = ctypes.c_uint32
+vp = ctypes.c_void_p
+
+
+class GalOp(ctypes.Structure):
+    _fields_ = [("kind", u32), ("slot", u32), ("arg", u64), ("tid", u64)]
+
+
+class GalInfo(ctypes.Structure):
+    _fields_ = [
+        ("segment_size", u64),
+        ("min_size", u64),
+        ("max_size", u64),
+        ("num_segments", u64),
+        ("blocks_per_segment", u64),
+        ("num_trees", u64),
+        ("veb_layers", u64),
+        ("total_slots", u64),
+        ("pool_device_ptr", u64),
+        ("device_handle", u64),
+        ("snapshot_bytes", u64),
+        ("metadata_bytes", u64),
+    ]
+
+
+class GalRunResult(ctypes.Structure):
+    _fields_ = [
+        ("mallocs", u64),
+        ("frees", u64),
+        ("misses", u64),
+        ("tag_errors", u64),
+        ("alloc_errors", u64),
+        ("launches", u64),
+        ("malloc_kernel_ms", ctypes.c_double),
+        ("free_kernel_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_SIGNATURES = {
+    "gal_supported": (ctypes.c_int, [ctypes.POINTER(u64), ctypes.c_int]),
+    "gal_create": (ctypes.c_int, [u64, u64, u64, u64, u64, ctypes.POINTER(vp)]),
+    "gal_destroy": (ctypes.c_int, [vp]),
+    "gal_get_info": (ctypes.c_int, [vp, ctypes.POINTER(GalInfo)]),
+    "gal_print_stats": (ctypes.c_int, [vp]),
+    "gal_snapshot": (ctypes.c_int, [vp, vp, u64, ctypes.POINTER(u64)]),
+    "gal_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64)]),
+    "gal_errors": (ctypes.c_int, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "gal_init_global_allocator": (ctypes.c_int, [u64, u64, ctypes.c_int]),
+    "gal_free_global_allocator": (ctypes.c_int, []),
+    "gal_print_global_stats": (ctypes.c_int, []),
+    "gal_global_smoke": (ctypes.c_int, [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "gal_run_plain": (ctypes.c_int, [vp, u64, u64, u64, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_mixed_reset": (ctypes.c_int, [vp, u64, u32, u64]),
+    "gal_mixed_round": (ctypes.c_int, [vp, u32, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_mixed_drain": (ctypes.c_int, [vp, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_mixed_live": (
+        ctypes.c_int,
+        [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), u64, ctypes.POINTER(u64)],
+    ),
+    "gal_run_mixed": (ctypes.c_int, [vp, u64, u32, u64, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_read_pool": (ctypes.c_int, [vp, u64, vp, u64]),
+    "gal_latency_samples": (
+        ctypes.c_int,
+        [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],
+    ),
+}
+
+
+class GallatinError(RuntimeError):
+    def __init__(self, func, code):
+        super().__init__(f"{func} failed: {ERROR_NAMES.get(code, code)} ({code})")
+        self.code = code
+
+
+def load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `make -C gallatin_amd/csrc`"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def check(func, code):
+    if code != GAL_OK:
+        raise GallatinError(func, code)
+
+
+lib = load()

@@ -1,0 +1,728 @@
+// capi.hip — the C-ABI of the MI355X Gallatin allocator (include/gallatin_amd.h).
+//
+// Templates cannot cross a C ABI, so each supported (segment, min, max)
+// triple is instantiated here once and reached through a table of function
+// pointers. All device work runs on the handle's own stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include <gallatin/allocators/global_allocator.hpp>
+#include <gallatin/workloads.hpp>
+
+#include "../../include/gallatin_amd.h"
+
+namespace ga = gallatin::allocators;
+namespace gw = gallatin::workloads;
+namespace gu = gallatin::utils;
+namespace wave = gallatin::wave;
+namespace atom = gallatin::atom;
+
+namespace {
+
+struct run_counters {
+  unsigned long long mallocs, frees, misses, tag_errors;
+};
+
+constexpr int kEventPairs = 96;
+
+struct impl_ops;
+
+}  // namespace
+
+struct gal_allocator {
+  const impl_ops* ops = nullptr;
+  void* dev = nullptr;  // Gallatin<...>* in device memory
+  gal_info_t info{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2 * kEventPairs] = {};
+  run_counters* d_counters = nullptr;
+  uint32_t* control = nullptr;  // device control words of the allocator
+  // mixed workload state
+  uint64_t* d_live = nullptr;
+  uint64_t live_cap = 0;
+  uint64_t threads_per_round = 0;
+  uint32_t rounds = 0;
+  uint32_t rounds_done = 0;
+  uint64_t mixed_seed = 0;
+  // per-wave malloc latency samples (device wall-clock ticks) of the last run
+  uint32_t* d_lat = nullptr;
+  uint64_t lat_cap = 0;
+  uint64_t lat_count = 0;
+  double tick_ns = 10.0;
+};
+
+namespace {
+
+struct impl_ops {
+  uint64_t seg, min, max;
+  void* (*create)(uint64_t pool, uint64_t seed);
+  void (*destroy)(void* dev);
+  int (*describe)(gal_allocator* h);
+  int (*snapshot)(gal_allocator* h, uint8_t* out);
+  int (*replay)(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t* d_results, uint32_t* d_status);
+  void (*launch_plain)(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify);
+  void (*launch_mixed_malloc)(gal_allocator* h, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
+                              uint32_t verify, uint32_t* lat);
+  void (*launch_mixed_free)(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
+                            bool all);
+  void (*print)(gal_allocator* h);
+};
+
+// ---------------------------------------------------------------- kernels
+__device__ inline bool tag_in(uint64_t* p, uint64_t tag, uint32_t verify) {
+  if (verify == GAL_VERIFY_ATOMIC) return atom::exch(p, tag) == 0ull;
+  *(volatile uint64_t*)p = tag;
+  return *(volatile uint64_t*)p == tag;
+}
+__device__ inline bool tag_out(uint64_t* p, uint64_t tag, uint32_t verify) {
+  if (verify == GAL_VERIFY_ATOMIC) return atom::exch(p, 0ull) == tag;
+  bool ok = *(volatile uint64_t*)p == tag;
+  *(volatile uint64_t*)p = 0ull;
+  return ok;
+}
+
+// wave-aggregated counter update: one atomic per wave and counter
+__device__ inline void count(run_counters* c, bool m, bool f, bool miss, bool bad) {
+  const uint64_t active = wave::active_mask();
+  const uint32_t nm = wave::popc(wave::ballot(m)), nf = wave::popc(wave::ballot(f));
+  const uint32_t nx = wave::popc(wave::ballot(miss)), nb = wave::popc(wave::ballot(bad));
+  if ((int)wave::lane_id() == wave::first_lane(active)) {
+    if (nm) atom::add(&c->mallocs, (unsigned long long)nm);
+    if (nf) atom::add(&c->frees, (unsigned long long)nf);
+    if (nx) atom::add(&c->misses, (unsigned long long)nx);
+    if (nb) atom::add(&c->tag_errors, (unsigned long long)nb);
+  }
+}
+
+template <class A>
+__global__ void replay_kernel(A* a, const gal_op* ops, uint64_t n, uint64_t* results, uint32_t* status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  char* base = a->table.memory;
+  for (uint64_t i = 0; i < n; i++) {
+    const gal_op op = ops[i];
+    if (op.kind == GAL_OP_MALLOC) {
+      void* p = a->malloc_at(op.arg, op.tid, op.slot);
+      results[i] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
+    } else {
+      const uint64_t r = results[op.arg];
+      if (r == GAL_NULL_OFFSET) {
+        *status = 1u;
+        return;
+      }
+      a->free(base + r);
+      results[i] = 0;
+    }
+  }
+}
+
+template <class A>
+__global__ void plain_kernel(A* a, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify, run_counters* c) {
+  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool got = false, bad = false, miss = false;
+  if (tid < n) {
+    uint64_t* p = (uint64_t*)a->malloc(size);
+    if (p) {
+      got = true;
+      const uint64_t tag = gw::payload_tag(seed, tid);
+      bad = !tag_in(p, tag, verify);
+      bad |= !tag_out(p, tag, verify);
+      a->free(p);
+    } else {
+      miss = true;
+    }
+  }
+  count(c, got, got, miss, bad);
+}
+
+// device wall clock (constant rate, see hipDeviceAttributeWallClockRate)
+__device__ inline uint64_t wall_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+template <class A>
+__global__ void mixed_malloc_kernel(A* a, uint64_t* live, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
+                                    uint32_t verify, run_counters* c, uint32_t* lat) {
+  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool got = false, bad = false, miss = false;
+  if (tid < n) {
+    const uint64_t t0 = wall_ticks();
+    uint64_t* p = (uint64_t*)a->malloc(gw::mixed_size(seed, round, tid));
+    const uint64_t t1 = wall_ticks();
+    // one latency sample per wave: the wave's time inside malloc
+    if (lat && (tid & 63) == 0) lat[(base + tid) >> 6] = (uint32_t)(t1 - t0);
+    live[base + tid] = (uint64_t)p;
+    if (p) {
+      got = true;
+      bad = !tag_in(p, gw::payload_tag(seed, base + tid), verify);
+    } else {
+      miss = true;
+    }
+  }
+  count(c, got, false, miss, bad);
+}
+
+template <class A>
+__global__ void mixed_free_kernel(A* a, uint64_t* live, uint64_t n_live, uint32_t round, uint64_t seed,
+                                  uint32_t verify, bool all, run_counters* c) {
+  const uint64_t j = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool freed = false, bad = false;
+  if (j < n_live) {
+    uint64_t* p = (uint64_t*)live[j];
+    if (p && (all || gw::mixed_coin(seed, round, j))) {
+      bad = !tag_out(p, gw::payload_tag(seed, j), verify);
+      a->free(p);
+      live[j] = 0;
+      freed = true;
+    }
+  }
+  count(c, false, freed, false, bad);
+}
+
+__global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) {
+  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool got = false, bad = false, miss = false;
+  if (tid < n) {
+    uint64_t* p = (uint64_t*)ga::global_malloc(size);
+    if (p) {
+      got = true;
+      const uint64_t tag = gw::payload_tag(7, tid);
+      bad = !tag_in(p, tag, GAL_VERIFY_ATOMIC);
+      bad |= !tag_out(p, tag, GAL_VERIFY_ATOMIC);
+      ga::global_free(p);
+    } else {
+      miss = true;
+    }
+  }
+  count(c, got, got, miss, bad);
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// ------------------------------------------------------ per-type impl
+template <uint64_t S, uint64_t M, uint64_t B>
+struct impl {
+  using A = ga::Gallatin<S, M, B>;
+
+  static void* create(uint64_t pool, uint64_t seed) { return (void*)A::generate_on_device(pool, seed, false); }
+  static void destroy(void* dev) { A::free_on_device((A*)dev); }
+
+  static int describe(gal_allocator* h) {
+    A host;
+    if (!A::host_copy((const A*)h->dev, &host)) return GAL_ERR_HIP;
+    gal_info_t& i = h->info;
+    std::memset(&i, 0, sizeof(i));
+    i.segment_size = S;
+    i.min_size = M;
+    i.max_size = B;
+    i.num_segments = host.table.num_segments;
+    i.blocks_per_segment = A::kBlocksPerSegment;
+    i.num_trees = A::kNumTrees;
+    i.veb_layers = (uint64_t)host.segment_tree.num_layers;
+    i.total_slots = A::total_slots();
+    i.pool_device_ptr = (uint64_t)host.table.memory;
+    i.device_handle = (uint64_t)h->dev;
+    uint64_t words = 8;
+    for (int l = 0; l < host.segment_tree.num_layers; l++) words += host.segment_tree.words[l] * (1 + A::kNumTrees);
+    const uint64_t nseg = i.num_segments, nblocks = nseg * A::kBlocksPerSegment;
+    uint64_t bytes = words * 8 + ((nseg * 2 + 7) / 8) * 8 + nblocks * 12 + nseg * 16 + i.total_slots * 4;
+    i.snapshot_bytes = ((bytes + 7) / 8) * 8;
+    i.metadata_bytes = words * 8 + nseg * 4 + nblocks * 12 + nseg * 16 + i.total_slots * 4 + sizeof(A);
+    h->control = host.control;
+    return GAL_OK;
+  }
+
+  static int snapshot(gal_allocator* h, uint8_t* out) {
+    A host;
+    if (!A::host_copy((const A*)h->dev, &host)) return GAL_ERR_HIP;
+    if (hipDeviceSynchronize() != hipSuccess) return GAL_ERR_HIP;
+    const uint64_t nseg = host.table.num_segments, nblocks = nseg * A::kBlocksPerSegment;
+    uint8_t* p = out;
+    uint32_t locks = 0;
+    if (hipMemcpy(&locks, host.control + ga::kCtlLocks, 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    uint64_t header[8] = {nseg, A::kBlocksPerSegment, A::kNumTrees, (uint64_t)host.segment_tree.num_layers,
+                          A::total_slots(), locks, 0, 0};
+    std::memcpy(p, header, sizeof(header));
+    p += sizeof(header);
+    auto tree = [&](const ga::veb_tree& t) {
+      for (int l = 0; l < t.num_layers; l++) {
+        if (hipMemcpy(p, t.bits[l], t.words[l] * 8, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        p += t.words[l] * 8;
+      }
+      return true;
+    };
+    if (!tree(host.segment_tree)) return GAL_ERR_HIP;
+    for (uint32_t c = 0; c < A::kNumTrees; c++)
+      if (!tree(host.sub_trees[c])) return GAL_ERR_HIP;
+    std::vector<uint32_t> ids(nseg);
+    if (hipMemcpy(ids.data(), host.table.chunk_ids, nseg * 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    for (uint64_t s = 0; s < nseg; s++) {
+      uint16_t v = (uint16_t)ids[s];
+      std::memcpy(p, &v, 2);
+      p += 2;
+    }
+    uint64_t pad = ((nseg * 2 + 7) / 8) * 8 - nseg * 2;
+    std::memset(p, 0, pad);
+    p += pad;
+    if (hipMemcpy(p, host.table.blocks, nblocks * 8, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    p += nblocks * 8;
+    if (hipMemcpy(p, host.table.queues, nblocks * 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    p += nblocks * 4;
+    if (hipMemcpy(p, host.table.segs, nseg * 16, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    p += nseg * 16;
+    if (hipMemcpy(p, host.slots, A::total_slots() * 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    p += A::total_slots() * 4;
+    uint64_t used = (uint64_t)(p - out);
+    if (h->info.snapshot_bytes > used) std::memset(p, 0, h->info.snapshot_bytes - used);
+    return GAL_OK;
+  }
+
+  static int replay(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t* d_results, uint32_t* d_status) {
+    replay_kernel<A><<<1, 1, 0, h->stream>>>((A*)h->dev, d_ops, n, d_results, d_status);
+    return hipGetLastError() == hipSuccess ? GAL_OK : GAL_ERR_HIP;
+  }
+
+  static void launch_plain(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify) {
+    plain_kernel<A><<<blocks_for(n, 256), 256, 0, h->stream>>>((A*)h->dev, n, size, seed, verify, h->d_counters);
+  }
+  static void launch_mixed_malloc(gal_allocator* h, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
+                                  uint32_t verify, uint32_t* lat) {
+    mixed_malloc_kernel<A><<<blocks_for(n, 256), 256, 0, h->stream>>>((A*)h->dev, h->d_live, base, n, round, seed,
+                                                                       verify, h->d_counters, lat);
+  }
+  static void launch_mixed_free(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
+                                bool all) {
+    if (n_live == 0) return;
+    mixed_free_kernel<A><<<blocks_for(n_live, 256), 256, 0, h->stream>>>((A*)h->dev, h->d_live, n_live, round, seed,
+                                                                         verify, all, h->d_counters);
+  }
+  static void print(gal_allocator* h) { ((A*)h->dev)->print_info(); }
+
+  static constexpr impl_ops ops() {
+    return impl_ops{S,       M,        B,           &create,          &destroy,
+                    &describe, &snapshot, &replay, &launch_plain, &launch_mixed_malloc,
+                    &launch_mixed_free, &print};
+  }
+};
+
+// The instantiation table: (segment bytes, smallest slice, largest slice).
+constexpr uint64_t MiB = 1ull << 20;
+const impl_ops kImpls[] = {
+    impl<16 * MiB, 16, 4096>::ops(),   // the reference's default and global allocator
+    impl<1 * MiB, 16, 256>::ops(),     // small segments: exhaustion and recycling tests
+    impl<4 * MiB, 128, 1024>::ops(),   // slab-hash tuning space (min 128 B)
+    impl<16 * MiB, 128, 4096>::ops(),
+    impl<64 * MiB, 128, 4096>::ops(),
+    impl<64 * MiB, 16, 16384>::ops(),  // 16 KiB pages as single slices
+};
+
+const impl_ops* find_impl(uint64_t seg, uint64_t min, uint64_t max) {
+  for (const auto& o : kImpls)
+    if (o.seg == seg && o.min == min && o.max == max) return &o;
+  return nullptr;
+}
+
+// events around launches on the handle's stream
+struct launch_timer {
+  gal_allocator* h;
+  int used = 0;
+  std::vector<int> kinds;  // 0 malloc kernel, 1 free-only kernel
+  explicit launch_timer(gal_allocator* hh) : h(hh) {}
+  bool begin() {
+    if (used >= kEventPairs) return false;
+    return hipEventRecord(h->ev[2 * used], h->stream) == hipSuccess;
+  }
+  void end(int kind) {
+    (void)hipEventRecord(h->ev[2 * used + 1], h->stream);
+    kinds.push_back(kind);
+    used++;
+  }
+  int finish(gal_run_result* out) {
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+    if (hipGetLastError() != hipSuccess) return GAL_ERR_HIP;
+    for (int i = 0; i < used; i++) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]) != hipSuccess) return GAL_ERR_HIP;
+      if (kinds[i] == 0)
+        out->malloc_kernel_ms += ms;
+      else
+        out->free_kernel_ms += ms;
+    }
+    if (used > 0) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, h->ev[0], h->ev[2 * used - 1]) != hipSuccess) return GAL_ERR_HIP;
+      out->total_ms += ms;
+    }
+    out->launches += (uint64_t)used;
+    return GAL_OK;
+  }
+};
+
+int read_errors(gal_allocator* h, uint64_t* internal, uint64_t* bad) {
+  uint32_t ctl[ga::kCtlWords];
+  if (hipMemcpy(ctl, h->control, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  if (internal) *internal = ctl[ga::kCtlErrors];
+  if (bad) *bad = ctl[ga::kCtlBadFrees];
+  return GAL_OK;
+}
+
+// run `body` (which launches through the timer) with zeroed counters; fill out
+template <class F>
+int timed_run(gal_allocator* h, gal_run_result* out, F&& body) {
+  gal_run_result r;
+  std::memset(&r, 0, sizeof(r));
+  uint64_t err0 = 0, err1 = 0;
+  if (read_errors(h, &err0, nullptr) != GAL_OK) return GAL_ERR_HIP;
+  if (hipMemsetAsync(h->d_counters, 0, sizeof(run_counters), h->stream) != hipSuccess) return GAL_ERR_HIP;
+  launch_timer t(h);
+  int rc = body(t);
+  if (rc != GAL_OK) {
+    (void)hipStreamSynchronize(h->stream);
+    return rc;
+  }
+  rc = t.finish(&r);
+  if (rc != GAL_OK) return rc;
+  run_counters c;
+  if (hipMemcpy(&c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  if (read_errors(h, &err1, nullptr) != GAL_OK) return GAL_ERR_HIP;
+  r.mallocs = c.mallocs;
+  r.frees = c.frees;
+  r.misses = c.misses;
+  r.tag_errors = c.tag_errors;
+  r.alloc_errors = err1 - err0;
+  if (out) *out = r;
+  return GAL_OK;
+}
+
+// state of the global allocator made by gal_init_global_allocator
+gal_allocator* g_global = nullptr;
+
+}  // namespace
+
+extern "C" {
+
+int gal_supported(uint64_t* out, int cap) {
+  int n = (int)(sizeof(kImpls) / sizeof(kImpls[0]));
+  for (int i = 0; i < n && i < cap && out; i++) {
+    out[3 * i] = kImpls[i].seg;
+    out[3 * i + 1] = kImpls[i].min;
+    out[3 * i + 2] = kImpls[i].max;
+  }
+  return n;
+}
+
+static int make_handle(const impl_ops* ops, void* dev, gal_allocator** out) {
+  gal_allocator* h = new gal_allocator();
+  h->ops = ops;
+  h->dev = dev;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return GAL_ERR_HIP;
+  }
+  for (auto& e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) return GAL_ERR_HIP;
+  if (hipMalloc(&h->d_counters, sizeof(run_counters)) != hipSuccess) return GAL_ERR_HIP;
+  int khz = 0;
+  int device = 0;
+  if (hipGetDevice(&device) == hipSuccess &&
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    h->tick_ns = 1e6 / (double)khz;
+  int rc = ops->describe(h);
+  if (rc != GAL_OK) return rc;
+  *out = h;
+  return GAL_OK;
+}
+
+int gal_create(uint64_t segment_size, uint64_t min_size, uint64_t max_size, uint64_t pool_bytes, uint64_t seed,
+               gal_allocator** out) {
+  if (!out) return GAL_ERR_INVALID;
+  *out = nullptr;
+  const impl_ops* ops = find_impl(segment_size, min_size, max_size);
+  if (!ops) return GAL_ERR_UNSUPPORTED;
+  if (pool_bytes < segment_size) return GAL_ERR_INVALID;
+  void* dev = ops->create(pool_bytes, seed);
+  if (!dev) return GAL_ERR_OOM;
+  gal_allocator* h = nullptr;
+  int rc = make_handle(ops, dev, &h);
+  if (rc != GAL_OK) {
+    ops->destroy(dev);
+    return rc;
+  }
+  *out = h;
+  return GAL_OK;
+}
+
+int gal_destroy(gal_allocator* h) {
+  if (!h) return GAL_ERR_INVALID;
+  (void)hipStreamSynchronize(h->stream);
+  if (h->dev) h->ops->destroy(h->dev);
+  if (h->d_live) (void)hipFree(h->d_live);
+  if (h->d_lat) (void)hipFree(h->d_lat);
+  if (h->d_counters) (void)hipFree(h->d_counters);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return GAL_OK;
+}
+
+int gal_get_info(gal_allocator* h, gal_info_t* info) {
+  if (!h || !info) return GAL_ERR_INVALID;
+  *info = h->info;
+  return GAL_OK;
+}
+
+int gal_print_stats(gal_allocator* h) {
+  if (!h) return GAL_ERR_INVALID;
+  (void)hipStreamSynchronize(h->stream);
+  h->ops->print(h);
+  fflush(stdout);
+  return GAL_OK;
+}
+
+int gal_snapshot(gal_allocator* h, void* buf, uint64_t cap, uint64_t* needed) {
+  if (!h) return GAL_ERR_INVALID;
+  if (needed) *needed = h->info.snapshot_bytes;
+  if (!buf || cap < h->info.snapshot_bytes) return GAL_ERR_BUFFER;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  return h->ops->snapshot(h, (uint8_t*)buf);
+}
+
+int gal_errors(gal_allocator* h, uint64_t* internal_errors, uint64_t* bad_frees) {
+  if (!h) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  return read_errors(h, internal_errors, bad_frees);
+}
+
+int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* results) {
+  if (!h || (n && (!ops || !results))) return GAL_ERR_INVALID;
+  if (n == 0) return GAL_OK;
+  // a free must name an earlier malloc op
+  for (uint64_t i = 0; i < n; i++) {
+    if (ops[i].kind == GAL_OP_FREE) {
+      if (ops[i].arg >= i || ops[ops[i].arg].kind != GAL_OP_MALLOC) return GAL_ERR_INVALID;
+    } else if (ops[i].kind != GAL_OP_MALLOC) {
+      return GAL_ERR_INVALID;
+    }
+  }
+  gal_op* d_ops = nullptr;
+  uint64_t* d_res = nullptr;
+  uint32_t* d_status = nullptr;
+  int rc = GAL_OK;
+  if (hipMalloc(&d_ops, n * sizeof(gal_op)) != hipSuccess || hipMalloc(&d_res, n * 8) != hipSuccess ||
+      hipMalloc(&d_status, 4) != hipSuccess) {
+    rc = GAL_ERR_OOM;
+  }
+  if (rc == GAL_OK && (hipMemcpy(d_ops, ops, n * sizeof(gal_op), hipMemcpyHostToDevice) != hipSuccess ||
+                       hipMemset(d_status, 0, 4) != hipSuccess))
+    rc = GAL_ERR_HIP;
+  if (rc == GAL_OK) rc = h->ops->replay(h, d_ops, n, d_res, d_status);
+  if (rc == GAL_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = GAL_ERR_HIP;
+  uint32_t status = 0;
+  if (rc == GAL_OK && (hipMemcpy(results, d_res, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                       hipMemcpy(&status, d_status, 4, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = GAL_ERR_HIP;
+  if (rc == GAL_OK && status != 0) rc = GAL_ERR_INVALID;  // freed a failed malloc
+  if (d_ops) (void)hipFree(d_ops);
+  if (d_res) (void)hipFree(d_res);
+  if (d_status) (void)hipFree(d_status);
+  return rc;
+}
+
+int gal_run_plain(gal_allocator* h, uint64_t n_threads, uint64_t size, uint64_t seed, uint32_t verify,
+                  gal_run_result* out) {
+  if (!h || n_threads == 0 || verify > GAL_VERIFY_ATOMIC) return GAL_ERR_INVALID;
+  return timed_run(h, out, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    h->ops->launch_plain(h, n_threads, size, seed, verify);
+    t.end(0);
+    return GAL_OK;
+  });
+}
+
+int gal_mixed_reset(gal_allocator* h, uint64_t threads_per_round, uint32_t rounds, uint64_t seed) {
+  if (!h || threads_per_round == 0 || rounds == 0) return GAL_ERR_INVALID;
+  // free whatever the previous run left live
+  if (h->d_live && h->rounds_done > 0) {
+    h->ops->launch_mixed_free(h, h->threads_per_round * h->rounds_done, 0, h->mixed_seed, GAL_VERIFY_STORE, true);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  }
+  // every slot of the live table is written by its round's malloc kernel
+  // before any free kernel reads it, so the table needs no clearing
+  const uint64_t need = threads_per_round * rounds;
+  if (need > h->live_cap) {
+    if (h->d_live) (void)hipFree(h->d_live);
+    h->d_live = nullptr;
+    h->live_cap = 0;
+    if (hipMalloc(&h->d_live, need * 8) != hipSuccess) return GAL_ERR_OOM;
+    h->live_cap = need;
+  }
+  const uint64_t lat_need = (need + 63) / 64 + 1;
+  if (lat_need > h->lat_cap) {
+    if (h->d_lat) (void)hipFree(h->d_lat);
+    h->d_lat = nullptr;
+    h->lat_cap = 0;
+    if (hipMalloc(&h->d_lat, lat_need * 4) != hipSuccess) return GAL_ERR_OOM;
+    h->lat_cap = lat_need;
+  }
+  h->lat_count = 0;
+  h->threads_per_round = threads_per_round;
+  h->rounds = rounds;
+  h->rounds_done = 0;
+  h->mixed_seed = seed;
+  return GAL_OK;
+}
+
+int gal_mixed_round(gal_allocator* h, uint32_t round, uint32_t verify, gal_run_result* out) {
+  const uint64_t seed = h ? h->mixed_seed : 0;
+  if (!h || !h->d_live || round != h->rounds_done || round >= h->rounds || verify > GAL_VERIFY_ATOMIC)
+    return GAL_ERR_INVALID;
+  int rc = timed_run(h, out, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    h->ops->launch_mixed_malloc(h, (uint64_t)round * h->threads_per_round, h->threads_per_round, round, seed, verify,
+                                h->d_lat);
+    t.end(0);
+    if (!t.begin()) return GAL_ERR_INVALID;
+    h->ops->launch_mixed_free(h, (uint64_t)(round + 1) * h->threads_per_round, round, seed, verify, false);
+    t.end(1);
+    return GAL_OK;
+  });
+  if (rc == GAL_OK) {
+    h->rounds_done++;
+    h->lat_count = ((uint64_t)h->rounds_done * h->threads_per_round + 63) / 64;
+  }
+  return rc;
+}
+
+int gal_mixed_drain(gal_allocator* h, uint32_t verify, gal_run_result* out) {
+  if (!h || !h->d_live || verify > GAL_VERIFY_ATOMIC) return GAL_ERR_INVALID;
+  const uint64_t seed = h->mixed_seed;
+  const uint64_t n_live = (uint64_t)h->rounds_done * h->threads_per_round;
+  int rc = timed_run(h, out, [&](launch_timer& t) {
+    if (n_live == 0) return GAL_OK;
+    if (!t.begin()) return GAL_ERR_INVALID;
+    h->ops->launch_mixed_free(h, n_live, 0, seed, verify, true);
+    t.end(1);
+    return GAL_OK;
+  });
+  if (rc == GAL_OK) h->rounds_done = 0;
+  return rc;
+}
+
+int gal_mixed_live(gal_allocator* h, uint64_t* offsets, uint64_t* sizes, uint64_t* slots, uint64_t cap,
+                   uint64_t* count) {
+  if (!h || !count) return GAL_ERR_INVALID;
+  const uint64_t n_live = (uint64_t)h->rounds_done * h->threads_per_round;
+  std::vector<uint64_t> ptrs(n_live);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  if (n_live && hipMemcpy(ptrs.data(), h->d_live, n_live * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return GAL_ERR_HIP;
+  uint64_t k = 0;
+  for (uint64_t j = 0; j < n_live; j++) {
+    if (!ptrs[j]) continue;
+    if (k < cap) {
+      if (offsets) offsets[k] = ptrs[j] - h->info.pool_device_ptr;
+      if (sizes) sizes[k] = gw::mixed_size(h->mixed_seed, j / h->threads_per_round, j % h->threads_per_round);
+      if (slots) slots[k] = j;
+    }
+    k++;
+  }
+  *count = k;
+  return (offsets && k > cap) ? GAL_ERR_BUFFER : GAL_OK;
+}
+
+int gal_run_mixed(gal_allocator* h, uint64_t threads_per_round, uint32_t rounds, uint64_t seed, uint32_t verify,
+                  gal_run_result* out) {
+  int rc = gal_mixed_reset(h, threads_per_round, rounds, seed);
+  if (rc != GAL_OK) return rc;
+  gal_run_result total;
+  std::memset(&total, 0, sizeof(total));
+  rc = timed_run(h, &total, [&](launch_timer& t) {
+    for (uint32_t r = 0; r < rounds; r++) {
+      if (!t.begin()) return GAL_ERR_INVALID;
+      h->ops->launch_mixed_malloc(h, (uint64_t)r * threads_per_round, threads_per_round, r, seed, verify, h->d_lat);
+      t.end(0);
+      if (!t.begin()) return GAL_ERR_INVALID;
+      h->ops->launch_mixed_free(h, (uint64_t)(r + 1) * threads_per_round, r, seed, verify, false);
+      t.end(1);
+    }
+    if (!t.begin()) return GAL_ERR_INVALID;
+    h->ops->launch_mixed_free(h, (uint64_t)rounds * threads_per_round, 0, seed, verify, true);
+    t.end(1);
+    return GAL_OK;
+  });
+  if (rc == GAL_OK) {
+    h->rounds_done = 0;  // drained
+    h->lat_count = ((uint64_t)rounds * threads_per_round + 63) / 64;
+  }
+  if (out) *out = total;
+  return rc;
+}
+
+int gal_latency_samples(gal_allocator* h, uint32_t* ticks, uint64_t cap, uint64_t* count, double* tick_ns) {
+  if (!h || !count) return GAL_ERR_INVALID;
+  *count = h->lat_count;
+  if (tick_ns) *tick_ns = h->tick_ns;
+  if (!ticks) return GAL_OK;
+  if (cap < h->lat_count) return GAL_ERR_BUFFER;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  if (h->lat_count && hipMemcpy(ticks, h->d_lat, h->lat_count * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return GAL_ERR_HIP;
+  return GAL_OK;
+}
+
+int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n) {
+  if (!h || !dst) return GAL_ERR_INVALID;
+  if (off + n > h->info.num_segments * h->info.segment_size) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  return hipMemcpy(dst, (const char*)h->info.pool_device_ptr + off, n, hipMemcpyDeviceToHost) == hipSuccess
+             ? GAL_OK
+             : GAL_ERR_HIP;
+}
+
+int gal_init_global_allocator(uint64_t num_bytes, uint64_t seed, int print_info) {
+  if (g_global) return GAL_ERR_INVALID;
+  if (!ga::init_global_allocator(num_bytes, seed, print_info != 0)) return GAL_ERR_OOM;
+  const impl_ops* ops = find_impl(16 * MiB, 16, 4096);
+  gal_allocator* h = nullptr;
+  int rc = make_handle(ops, (void*)ga::get_global_allocator(), &h);
+  if (rc != GAL_OK) {
+    ga::free_global_allocator();
+    return rc;
+  }
+  g_global = h;
+  return GAL_OK;
+}
+
+int gal_free_global_allocator(void) {
+  if (!g_global) return GAL_ERR_INVALID;
+  ga::free_global_allocator();
+  g_global->dev = nullptr;  // already released
+  gal_destroy(g_global);
+  g_global = nullptr;
+  return GAL_OK;
+}
+
+int gal_print_global_stats(void) {
+  if (!g_global) return GAL_ERR_INVALID;
+  ga::print_global_stats();
+  fflush(stdout);
+  return GAL_OK;
+}
+
+int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors) {
+  if (!g_global || n_threads == 0) return GAL_ERR_INVALID;
+  gal_run_result r;
+  int rc = timed_run(g_global, &r, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    global_smoke_kernel<<<blocks_for(n_threads, 256), 256, 0, g_global->stream>>>(n_threads, size,
+                                                                                   g_global->d_counters);
+    t.end(0);
+    return GAL_OK;
+  });
+  if (misses) *misses = r.misses;
+  if (tag_errors) *tag_errors = r.tag_errors;
+  return rc;
+}
+
+}  // extern "C"

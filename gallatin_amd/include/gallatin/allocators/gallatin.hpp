@@ -1,0 +1,688 @@
+// gallatin.hpp — Gallatin<bytes_per_segment, smallest, biggest> for MI355X.
+//
+// Drop-in for the reference's allocator (include/gallatin/allocators/
+// gallatin.cuh:292-2035): host construction with generate_on_device /
+// free_on_device / print_info, and in-kernel malloc / free callable from any
+// lane under any divergence.
+//
+// A request of `size` bytes goes one of three ways (reference :1186-1276):
+//  * slice:   size <= biggest          -> one slice of class ceil(log2(size))-log2(smallest)
+//             size <= smallest*2048    -> 2^k contiguous slices of the top class
+//  * block:   size <= biggest*4096     -> a whole 4096-slice block of class
+//                                         ceil(log2(size)) - log2(smallest*4096)
+//  * segment: larger                   -> ceil(size / bytes_per_segment) contiguous segments
+// Slices are bump-claimed from per-class "pinned" blocks. Lanes of one
+// wavefront asking for the same class form a group: the group's last lane
+// makes ONE atomic add on the pinned block's malloc counter for the whole
+// group, and each lane takes its exclusive prefix of the group's slice count.
+// Frees are grouped the same way by block: one atomic add per (wave, block).
+// A block whose 4096 slices have all come back goes to its segment's recycle
+// queue; a segment whose blocks have all come back goes to the segment tree.
+#pragma once
+
+#include "device_utils.hpp"
+#include "memory_table.hpp"
+#include "veb.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace gallatin {
+namespace allocators {
+
+// Retry bounds. The reference's values (gallatin.cuh:108-112) where a retry
+// can see new state; out-of-memory retries are capped lower because with one
+// lane (or no concurrent frees) a retry repeats an identical failure.
+constexpr int kMallocLoopAttempts = 10;     // GALLATIN_MALLOC_LOOP_ATTEMPTS
+constexpr int kSliceAttempts = 500;         // GALLATIN_MAX_ATTEMPTS
+constexpr int kRequestBlockAttempts = 1000; // REQUEST_BLOCK_MAX_ATTEMPTS
+constexpr int kRequestBlockOomAttempts = 2; // failures with no segment anywhere
+constexpr int kBlockAttempts = 4;           // GALLATIN_MALLOC_BLOCK_ATTEMPTS (500) on OOM
+constexpr int kSegmentAttempts = 4;         // GALLATIN_MALLOC_SEGMENT_ATTEMPTS (500) on OOM
+constexpr int kPinnedWalk = 30;             // SHARED_BLOCK_COUNTER_CUTOFF
+constexpr uint32_t kPinnedSlotsClass0 = 128;
+constexpr uint32_t kPinnedSlotsMin = 4;     // MIN_PINNED_CUTOFF
+
+__host__ __device__ constexpr int ilog2(uint64_t x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+__host__ __device__ constexpr bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+// pinned slots of class c: 128 >> c, at least 4 (reference
+// shared_block_storage.cuh:251-274)
+__host__ __device__ constexpr uint32_t pinned_slots_of(uint32_t c) {
+  return (kPinnedSlotsClass0 >> (c > 31 ? 31 : c)) < kPinnedSlotsMin ? kPinnedSlotsMin : (kPinnedSlotsClass0 >> c);
+}
+
+template <uint64_t bytes_per_segment, uint64_t smallest, uint64_t biggest>
+struct Gallatin {
+  using my_type = Gallatin<bytes_per_segment, smallest, biggest>;
+  using table_type = alloc_table<bytes_per_segment, smallest>;
+
+  static_assert(is_pow2(bytes_per_segment) && is_pow2(smallest) && is_pow2(biggest), "powers of two");
+  static_assert(smallest <= biggest, "smallest <= biggest");
+  static_assert(bytes_per_segment >= biggest * 4096, "a segment holds at least one block of the top class");
+
+  static constexpr uint32_t kNumTrees = (uint32_t)(ilog2(biggest) - ilog2(smallest) + 1);
+  static constexpr int kSmallestBits = ilog2(smallest);
+  static constexpr int kSmallestTreeBits = ilog2(smallest * 4096);
+  static constexpr uint64_t kBlocksPerSegment = table_type::blocks_per_segment;
+  static_assert(kNumTrees <= 30, "class lock bits and the segment lock fit one u32");
+
+  static constexpr uint32_t total_slots() {
+    uint32_t t = 0;
+    for (uint32_t c = 0; c < kNumTrees; c++) t += pinned_slots_of(c);
+    return t;
+  }
+
+  // ---- device-resident state (immutable after construction except through
+  // the pointers; read with ordinary loads)
+  veb_tree segment_tree;          // free segments
+  veb_tree sub_trees[kNumTrees];  // segments formatted for each class
+  table_type table;
+  uint32_t* slots;                // pinned blocks, class-major: block id + 1
+  uint32_t slot_offset[kNumTrees];
+  uint32_t* control;              // control_word array
+  uint64_t seed;
+  // device allocations, for free_on_device
+  void* owned[6];
+
+#if defined(__HIPCC__)
+  // ------------------------------------------------------------ size classes
+  __host__ __device__ static constexpr uint32_t num_trees() { return kNumTrees; }
+  __device__ static uint32_t get_tree_id_from_size(uint64_t size) {
+    if (size < smallest) return 0;
+    return (uint32_t)(uint16_t)(utils::get_first_bit_bigger(size) - kSmallestBits);
+  }
+
+  __device__ uint32_t* lock_word() const { return &control[kCtlLocks]; }
+  __device__ bool acquire_tree_lock(uint32_t tree) {
+    uint32_t m = 1u << tree;
+    return (atom::fetch_or(lock_word(), m) & m) == 0;
+  }
+  __device__ void release_tree_lock(uint32_t tree) { atom::fetch_and(lock_word(), ~(1u << tree)); }
+  __device__ void count_error() { atom::add(&control[kCtlErrors], 1u); }
+
+  __device__ uint32_t* slot_ptr(uint32_t tree, uint32_t index) const { return &slots[slot_offset[tree] + index]; }
+
+  // start at hint % n and step past empty slots, at most 30 steps
+  // (reference shared_block_storage.cuh:119-134)
+  __device__ uint32_t valid_block_index(uint32_t tree, uint32_t hint) const {
+    const int n = (int)pinned_slots_of(tree);
+    int my = (int)(hint % (uint32_t)n);
+    const int original = my;
+    for (int counter = 0; atom::load(slot_ptr(tree, (uint32_t)my)) == 0u && my != original - 1;) {
+      my = (my + 1) % n;
+      if (++counter >= kPinnedWalk) break;
+    }
+    return (uint32_t)my;
+  }
+
+  // ------------------------------------------------- segments and blocks
+  // attach the lowest free segment to class `tree` (reference :1465-1510)
+  __device__ __noinline__ int64_t gather_new_segment(uint32_t tree) {
+    uint64_t segment = segment_tree.malloc_first();
+    if (segment == utils::kFail) return -1;
+    if (!table.setup_segment(segment, tree)) {
+      count_error();
+      return -1;
+    }
+    atom::fence();
+    sub_trees[tree].insert_force_update(segment);
+    atom::fence();
+    return (int64_t)segment;
+  }
+
+  // find a block for class `tree`, attaching a segment when the class has
+  // none with free blocks (reference :1525-1631); tid seeds the hashed search
+  __device__ __noinline__ int64_t request_new_block_from_tree(uint32_t tree, uint64_t tid) {
+    int oom = 0;
+    for (int attempts = 0; attempts < kRequestBlockAttempts; attempts++) {
+      uint64_t segment = sub_trees[tree].find_random_valid_index(tid);
+      if (segment == utils::kFail) {
+        if (acquire_tree_lock(tree)) {
+          int64_t got = gather_new_segment(tree);
+          release_tree_lock(tree);
+          if (got == -1) {
+            if (++oom >= kRequestBlockOomAttempts) return -1;
+            continue;
+          }
+          segment = (uint64_t)got;
+        } else {
+          wave::backoff(attempts);  // another lane is attaching a segment
+          continue;
+        }
+      }
+      atom::fence_acquire();
+      bool last = false;
+      int64_t block = table.get_block(segment, tree, last, control);
+      if (last) {
+        sub_trees[tree].remove(segment);
+        if (acquire_tree_lock(tree)) {
+          gather_new_segment(tree);
+          release_tree_lock(tree);
+        }
+        atom::fence();
+      }
+      if (block >= 0) return block;
+    }
+    return -1;
+  }
+
+  // return a fully freed block to its segment (reference :1637-1726)
+  __device__ __noinline__ void return_block(uint64_t block, uint64_t segment, uint32_t tree) {
+    const uint64_t nb = table_type::blocks_in_segment(tree);
+    const uint32_t slot = table.reserve_segment_slot(block, segment, nb, control);
+    // 1.0*slot/nb >= 0.1 && 1.0*(slot-1)/nb < 0.1 for power-of-two nb
+    // (reference :1646): re-list the segment once a tenth of it is back
+    if (slot >= 1 && 10ull * slot > nb && 10ull * (slot - 1) < nb) {
+      sub_trees[tree].insert_force_update(segment);
+      atom::fence();
+    }
+    const bool release = table.finish_freeing_block(segment, nb);
+    atom::fence();
+    if (release) {
+      sub_trees[tree].remove(segment);
+      atom::fence();
+      if (!table.reset_tree_id(segment, tree)) count_error();
+      atom::fence();
+      segment_tree.insert_force_update(segment);
+    }
+  }
+
+  __device__ void free_block(uint64_t block) {
+    const uint64_t segment = block / kBlocksPerSegment;
+    return_block(block, segment, table.read_tree_id(segment));
+  }
+
+  // account n returned slices of `block`; the return that completes 4096
+  // recycles the block (reference block.cuh:87-98, gallatin.cuh:1790-1799)
+  __device__ void free_slices(uint64_t block, uint32_t n) {
+    Block* b = &table.blocks[block];
+    uint32_t old = atom::add_release(&b->free_counter, n);
+    if (old + n == kSlicesPerBlock) {
+      atom::exch(&b->free_counter, 0u);
+      free_block(block);
+    }
+  }
+
+  // swap a spent pinned block for a fresh one (reference :1083-1138); a
+  // refill that loses the race hands its block back
+  __device__ __noinline__ bool replace_block(uint32_t tree, uint32_t index, uint64_t block, uint64_t tid) {
+    uint32_t* s = slot_ptr(tree, index);
+    if (atom::cas(s, (uint32_t)(block + 1), 0u) != (uint32_t)(block + 1)) return true;
+    atom::fence();
+    int64_t fresh = request_new_block_from_tree(tree, tid);
+    if (fresh < 0) return false;
+    if (atom::cas(s, 0u, (uint32_t)(fresh + 1)) != 0u) {
+      free_block((uint64_t)fresh);
+      return false;
+    }
+    return true;
+  }
+
+  // an empty pinned slot (left so by an out-of-memory replace) is refilled
+  __device__ __noinline__ bool refill_slot(uint32_t tree, uint32_t index, uint64_t tid) {
+    int64_t fresh = request_new_block_from_tree(tree, tid);
+    if (fresh < 0) return false;
+    if (atom::cas(slot_ptr(tree, index), 0u, (uint32_t)(fresh + 1)) != 0u) free_block((uint64_t)fresh);
+    return true;
+  }
+
+  // ------------------------------------------------------- slice claims
+  // One claim for the lanes of the current exec mask (all of class `tree`).
+  // Sets done/result for the lanes that are finished.
+  __device__ void claim_group(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint, uint64_t& result,
+                              bool& done) {
+    const uint64_t group = wave::active_mask();
+    const int leader = wave::first_lane(group);
+    const bool is_leader = (int)wave::lane_id() == leader;
+    const uint32_t lead_hint = wave::readfirst(hint);
+    const uint64_t lead_tid = wave::readfirst64(tid);
+
+    const uint32_t index = valid_block_index(tree, lead_hint);
+    const uint32_t entry = wave::readfirst(atom::load(slot_ptr(tree, index)));
+    if (entry == 0u) {
+      bool ok = false;
+      if (is_leader) ok = refill_slot(tree, index, lead_tid);
+      if (wave::ballot(ok) == 0) {  // class out of memory
+        done = true;
+        result = utils::kFail;
+      }
+      return;
+    }
+    const uint64_t block = entry - 1u;
+    Block* b = &table.blocks[block];
+
+    // exclusive scan of the slice counts in lane order
+    uint32_t prefix, total;
+    if (wave::ballot(count != 1u) == 0) {
+      prefix = wave::popc(group & wave::lanemask_lt());
+      total = wave::popc(group);
+    } else {
+      wave::group_scan(group, count, 13, prefix, total);
+    }
+    // the group's last lane holds the inclusive total and does the one add
+    // (reference block.cuh:245-264)
+    const int last = wave::last_lane(group);
+    uint32_t merged = 0;
+    if ((int)wave::lane_id() == last) merged = atom::add(&b->malloc_counter, total);
+    merged = wave::readlane(merged, last);
+
+    const uint32_t c0 = merged & kBlockCountMask;
+    uint64_t allocation = (uint64_t)c0 + prefix;
+    const bool should_replace = allocation <= 4095 && allocation + count > 4095;
+    const bool valid = allocation + count <= kSlicesPerBlock;
+    // Slices of the claimed range that lie inside the block but belong to no
+    // successful lane (the extra slices of multi-slice requests and a
+    // straddling request's tail) are written off as already freed
+    // (reference block.cuh:267-281): in-range slices minus successful lanes.
+    const uint32_t in_range = c0 >= kSlicesPerBlock ? 0u : ((c0 + total < kSlicesPerBlock ? c0 + total : kSlicesPerBlock) - c0);
+    const uint32_t written_off = in_range - wave::popc(wave::ballot(valid));
+    bool completed = false;
+    if (written_off > 0 && is_leader) {
+      uint32_t old = atom::add(&b->free_counter, written_off);
+      completed = (old + written_off == kSlicesPerBlock);
+    }
+    completed = wave::ballot(completed) != 0;
+    if (wave::ballot(should_replace) != 0 && is_leader) replace_block(tree, index, block, lead_tid);
+    if (completed && is_leader) {  // the write-off finished the block: recycle it
+      atom::exch(&b->free_counter, 0u);
+      free_block(block);
+    }
+    if ((merged >> kBlockTreeShift) != tree) {
+      // a stale pinned read: the block now serves another class; give back
+      // what the valid lanes drew (reference gallatin.cuh:1037-1056)
+      const uint32_t drawn = wave::popc(wave::ballot(valid));
+      if (drawn > 0 && is_leader) free_slices(block, drawn);
+      return;
+    }
+    if (valid) {
+      result = allocation + block * kSlicesPerBlock;
+      done = true;
+    }
+  }
+
+  // reference gallatin.cuh:865-1078 for every lane of the wave that calls it
+  __device__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint) {
+    uint64_t result = utils::kFail;
+    bool done = false;
+    for (int attempt = 0; attempt < kSliceAttempts; attempt++) {
+      for (;;) {  // one claim per distinct class among the lanes still looking
+        const uint32_t lead_tree = wave::readfirst(tree);
+        if (tree == lead_tree) {
+          claim_group(lead_tree, count, tid, hint, result, done);
+          break;
+        }
+      }
+      if (done) break;
+    }
+    return result;
+  }
+
+  // whole-block request (reference :828-861 + Block::malloc_fill_block)
+  __device__ uint64_t malloc_block_allocation(uint32_t tree, uint64_t tid) {
+    int64_t block = request_new_block_from_tree(tree, tid);
+    if (block < 0) return utils::kFail;
+    Block* b = &table.blocks[block];
+    uint32_t old_merged = atom::add(&b->malloc_counter, kSlicesPerBlock);
+    uint32_t old_count = old_merged & kBlockCountMask;
+    if ((old_merged >> kBlockTreeShift) != tree) count_error();
+    atom::add(&b->free_counter, 4095u - old_count);
+    if (old_count != 0) {
+      free_slices((uint64_t)block, 1u);
+      return utils::kFail;
+    }
+    return (uint64_t)block * kSlicesPerBlock;
+  }
+
+  // contiguous segments under the segment lock (reference :790-826)
+  __device__ uint64_t malloc_segment_allocation(uint64_t n) {
+    for (int spin = 0; !acquire_tree_lock(kNumTrees); spin++) {
+      if (spin > (1 << 22)) return utils::kFail;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    atom::fence_acquire();
+    uint64_t first = segment_tree.gather_multiple(n);
+    if (first != utils::kFail) {
+      if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error();
+    }
+    atom::fence_release();
+    release_tree_lock(kNumTrees);
+    return first == utils::kFail ? utils::kFail : first * kBlocksPerSegment * kSlicesPerBlock;
+  }
+
+  // memory_table.cuh:1072-1086
+  __device__ void* offset_to_allocation(uint64_t offset, uint32_t tree) const {
+    const uint64_t per = table_type::max_allocations_per_segment;
+    return table.memory + (offset / per) * bytes_per_segment + (offset % per) * table_type::tree_alloc_size(tree);
+  }
+
+  // --------------------------------------------------------------- malloc
+  // malloc with the slot selector and hash id given explicitly; malloc()
+  // passes the hardware CU id and the global thread id.
+  __device__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint) {
+    uint32_t tree = get_tree_id_from_size(size);
+    uint32_t count = 1;
+    if (tree >= kNumTrees) {
+      const int block_tree = utils::get_first_bit_bigger(size) - kSmallestTreeBits;
+      if (block_tree < 0) {
+        count = 1u << (tree - (kNumTrees - 1));
+        tree = kNumTrees - 1;
+      } else {
+        // block and segment requests: one lane of the wave at a time
+        const bool is_block = (uint32_t)block_tree < kNumTrees;
+        const uint64_t n = is_block ? 0 : (size - 1) / bytes_per_segment + 1;
+        void* out = nullptr;
+        for (uint64_t pending = wave::active_mask(); pending; pending &= pending - 1) {
+          if ((int)wave::lane_id() == wave::first_lane(pending)) {
+            uint64_t offset = utils::kFail;
+            if (is_block) {
+              for (int a = 0; a < kBlockAttempts && offset == utils::kFail; a++)
+                offset = malloc_block_allocation((uint32_t)block_tree, tid);
+              if (offset != utils::kFail) out = offset_to_allocation(offset, (uint32_t)block_tree);
+            } else if (n + kNumTrees + 1 < 0xFFFFull) {
+              for (int a = 0; a < kSegmentAttempts && offset == utils::kFail; a++)
+                offset = malloc_segment_allocation(n);
+              if (offset != utils::kFail) out = offset_to_allocation(offset, 0);
+            }
+          }
+        }
+        return out;
+      }
+    }
+    uint64_t offset = utils::kFail;
+    for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail; attempt++)
+      offset = malloc_slice_allocation(tree, count, tid, hint);
+    if (offset == utils::kFail) return nullptr;
+    return offset_to_allocation(offset, tree);
+  }
+
+  // reference gallatin.cuh:1186
+  __device__ void* malloc(uint64_t size) {
+    const uint64_t tid = (uint64_t)(uint32_t)(threadIdx.x + blockIdx.x * blockDim.x);
+    return malloc_at(size, tid, __smid());
+  }
+
+  // ----------------------------------------------------------------- free
+  // slices of one block freed together by the lanes of a wave
+  __device__ void free_offset(uint64_t offset) {
+    const uint64_t block = offset / kSlicesPerBlock;
+    for (;;) {
+      const uint64_t lead = wave::readfirst64(block);
+      if (block == lead) {
+        const uint64_t group = wave::active_mask();
+        if ((int)wave::lane_id() == wave::first_lane(group)) free_slices(lead, wave::popc(group));
+        break;
+      }
+    }
+  }
+
+  // reference gallatin.cuh:1378-1461
+  __device__ void free(void* allocation) {
+    const uint64_t byte = (uint64_t)((char*)allocation - table.memory);
+    const uint64_t segment = byte / bytes_per_segment;
+    if (byte >= table.num_segments * bytes_per_segment) {
+      atom::add(&control[kCtlBadFrees], 1u);
+      return;
+    }
+    const uint32_t tree = table.read_tree_id(segment);
+    if (tree == kFreeChunk || tree == kNumTrees || tree == kNumTrees + 1) {
+      atom::add(&control[kCtlBadFrees], 1u);
+      return;
+    }
+    if (tree > kNumTrees) {
+      // a run of segments: release the id first so that no claimer can find
+      // the segment in the tree while the old id is still set
+      if (!table.reset_tree_id(segment, tree)) count_error();
+      atom::fence_release();
+      segment_tree.return_multiple(segment, tree - kNumTrees - 1);
+      return;
+    }
+    const uint64_t offset =
+        (byte - segment * bytes_per_segment) / table_type::tree_alloc_size(tree) +
+        segment * table_type::max_allocations_per_segment;
+    free_offset(offset);
+  }
+
+  __device__ bool owns_allocation(void* allocation) const {
+    return (uint64_t)((char*)allocation - table.memory) < table.num_segments * bytes_per_segment;
+  }
+
+  // ------------------------------------------------------------ boot
+  // fill every pinned slot in class-major order with hash id 0
+  // (reference boot_shared_block_container_one_thread, gallatin.cuh:196-221)
+  __device__ bool boot_pinned_blocks() {
+    for (uint32_t tree = 0; tree < kNumTrees; tree++) {
+      for (uint32_t s = 0; s < pinned_slots_of(tree); s++) {
+        int64_t block = request_new_block_from_tree(tree, 0);
+        if (block < 0) return false;
+        if (table.read_tree_id((uint64_t)block / kBlocksPerSegment) != tree) return false;
+        if (atom::cas(slot_ptr(tree, s), 0u, (uint32_t)(block + 1)) != 0u) return false;
+      }
+    }
+    return true;
+  }
+#endif  // __HIPCC__
+
+  // ------------------------------------------------------------ host side
+  __host__ static uint64_t layer_words(uint64_t universe, int layers, uint64_t* words, uint64_t* universes) {
+    uint64_t u = universe, total = 0;
+    for (int i = layers - 1; i >= 0; i--) {
+      universes[i] = u;
+      words[i] = (u - 1) / 64 + 1;
+      total += words[i];
+      u = (u - 1) / 64 + 1;
+    }
+    return total;
+  }
+
+  // reference gallatin.cuh:324-421. Returns a device pointer, nullptr on failure.
+  __host__ static my_type* generate_on_device(uint64_t max_bytes, uint64_t seed, bool print_info = true,
+                                              bool running_calloc = false);
+  __host__ static void free_on_device(my_type* dev_version);
+  // called on the device pointer, as in the reference (:1855)
+  __host__ void print_info();
+  __host__ static bool host_copy(const my_type* dev, my_type* out) {
+    return hipMemcpy(out, dev, sizeof(my_type), hipMemcpyDeviceToHost) == hipSuccess;
+  }
+};
+
+#if defined(__HIPCC__)
+// ---- construction kernels
+static __global__ void gallatin_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+static __global__ void gallatin_init_blocks(Block* b, uint64_t n) {
+  for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    b[i] = Block{4097u, 0u};  // Block::init, reference block.cuh:64-68
+}
+static __global__ void gallatin_init_segments(segment_counters* s, uint64_t n) {
+  for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    s[i] = segment_counters{-1, 0u, 0u, 0u};
+}
+// item layer and summaries all set for the first `universe` items of each layer
+static __global__ void gallatin_init_full_layer(uint64_t* w, uint64_t words, uint64_t universe) {
+  for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < words; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo = i * 64;
+    uint64_t n = universe > lo ? universe - lo : 0;
+    w[i] = utils::bitmask(n > 64 ? 64 : n);
+  }
+}
+template <class A>
+__global__ void gallatin_boot_kernel(A* alloc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bool ok = alloc->boot_pinned_blocks();
+    atom::store(&alloc->control[kCtlBootStatus], ok ? 1u : 2u);
+  }
+}
+#endif
+
+namespace detail {
+__host__ inline unsigned grid_for(uint64_t n) {
+  uint64_t g = (n + 255) / 256;
+  return (unsigned)(g == 0 ? 1 : (g > 65535 ? 65535 : g));
+}
+}  // namespace detail
+
+template <uint64_t S, uint64_t M, uint64_t B>
+__host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_bytes, uint64_t seed, bool print_info,
+                                                         bool running_calloc) {
+  (void)running_calloc;  // calloc needs device-side launches; not supported
+  const uint64_t nseg = max_bytes / S;
+  if (nseg == 0) return nullptr;
+  const int layers = veb_tree::layers_for(nseg);
+  if (layers > kMaxVebLayers) return nullptr;
+
+  my_type host;
+  std::memset(&host, 0, sizeof(host));
+  host.seed = seed;
+
+  uint64_t words[kMaxVebLayers] = {0}, universes[kMaxVebLayers] = {0};
+  const uint64_t tree_words = layer_words(nseg, layers, words, universes);
+  const uint64_t nblocks = nseg * kBlocksPerSegment;
+
+  void* veb = nullptr;
+  void* table_mem = nullptr;
+  void* pool = nullptr;
+  void* dev = nullptr;
+  const size_t veb_bytes = (size_t)tree_words * 8 * (1 + kNumTrees);
+  // chunk ids | blocks | queues | segment counters | slots | control
+  const size_t off_blocks = ((nseg * 4 + 255) / 256) * 256;
+  const size_t off_queues = off_blocks + nblocks * sizeof(Block);
+  const size_t off_segs = off_queues + ((nblocks * 4 + 255) / 256) * 256;
+  const size_t off_slots = off_segs + nseg * sizeof(segment_counters);
+  const size_t off_ctl = off_slots + ((total_slots() * 4 + 255) / 256) * 256;
+  const size_t table_bytes = off_ctl + kCtlWords * 4;
+
+  auto fail = [&]() -> my_type* {
+    if (veb) (void)hipFree(veb);
+    if (table_mem) (void)hipFree(table_mem);
+    if (pool) (void)hipFree(pool);
+    if (dev) (void)hipFree(dev);
+    return nullptr;
+  };
+  if (hipMalloc(&veb, veb_bytes) != hipSuccess) return fail();
+  if (hipMalloc(&table_mem, table_bytes) != hipSuccess) return fail();
+  if (hipMalloc(&pool, nseg * S) != hipSuccess) return fail();
+  if (hipMalloc(&dev, sizeof(my_type)) != hipSuccess) return fail();
+
+  // vEB trees: segment tree full, class trees empty
+  uint64_t* vw = (uint64_t*)veb;
+  if (hipMemset(veb, 0, veb_bytes) != hipSuccess) return fail();
+  auto place = [&](veb_tree& t, uint64_t* base, uint64_t tree_seed) {
+    t.num_layers = layers;
+    t.seed = tree_seed;
+    t.total_universe = nseg;
+    uint64_t off = 0;
+    for (int i = 0; i < layers; i++) {
+      t.bits[i] = base + off;
+      t.words[i] = words[i];
+      t.universe[i] = universes[i];
+      off += words[i];
+    }
+  };
+  place(host.segment_tree, vw, seed);
+  for (int i = 0; i < layers; i++)
+    gallatin_init_full_layer<<<detail::grid_for(words[i]), 256>>>(host.segment_tree.bits[i], words[i], universes[i]);
+  for (uint32_t c = 0; c < kNumTrees; c++) place(host.sub_trees[c], vw + tree_words * (1 + c), seed + c);
+
+  char* tm = (char*)table_mem;
+  host.table.chunk_ids = (uint32_t*)tm;
+  host.table.blocks = (Block*)(tm + off_blocks);
+  host.table.queues = (uint32_t*)(tm + off_queues);
+  host.table.segs = (segment_counters*)(tm + off_segs);
+  host.table.memory = (char*)pool;
+  host.table.num_segments = nseg;
+  host.slots = (uint32_t*)(tm + off_slots);
+  host.control = (uint32_t*)(tm + off_ctl);
+  uint32_t off = 0;
+  for (uint32_t c = 0; c < kNumTrees; c++) {
+    host.slot_offset[c] = off;
+    off += pinned_slots_of(c);
+  }
+  if (hipMemset(table_mem, 0, table_bytes) != hipSuccess) return fail();
+  gallatin_fill_u32<<<detail::grid_for(nseg), 256>>>(host.table.chunk_ids, nseg, kFreeChunk);
+  gallatin_init_blocks<<<detail::grid_for(nblocks), 256>>>(host.table.blocks, nblocks);
+  gallatin_init_segments<<<detail::grid_for(nseg), 256>>>(host.table.segs, nseg);
+  if (hipMemset(pool, 0, nseg * S) != hipSuccess) return fail();
+
+  host.owned[0] = veb;
+  host.owned[1] = table_mem;
+  host.owned[2] = pool;
+  host.owned[3] = dev;
+  if (hipMemcpy(dev, &host, sizeof(my_type), hipMemcpyHostToDevice) != hipSuccess) return fail();
+  gallatin_boot_kernel<my_type><<<1, 1>>>((my_type*)dev);
+  if (hipDeviceSynchronize() != hipSuccess) return fail();
+  uint32_t status = 0;
+  if (hipMemcpy(&status, host.control + kCtlBootStatus, 4, hipMemcpyDeviceToHost) != hipSuccess || status != 1u)
+    return fail();
+  if (print_info)
+    printf("Booted Gallatin with %u trees in range %llu-%llu and %f GB of memory %llu segments\n", kNumTrees,
+           (unsigned long long)M, (unsigned long long)B, 1.0 * (double)max_bytes / 1024 / 1024 / 1024,
+           (unsigned long long)nseg);
+  return (my_type*)dev;
+}
+
+template <uint64_t S, uint64_t M, uint64_t B>
+__host__ void Gallatin<S, M, B>::free_on_device(my_type* dev_version) {
+  if (!dev_version) return;
+  my_type host;
+  if (!host_copy(dev_version, &host)) return;
+  (void)hipDeviceSynchronize();
+  for (int i = 0; i < 3; i++)
+    if (host.owned[i]) (void)hipFree(host.owned[i]);
+  (void)hipFree(dev_version);
+}
+
+template <uint64_t S, uint64_t M, uint64_t B>
+__host__ void Gallatin<S, M, B>::print_info() {
+  my_type host;
+  if (!host_copy(this, &host)) return;
+  (void)hipDeviceSynchronize();
+  auto fill = [&](const veb_tree& t) {
+    int b = t.num_layers - 1;
+    std::vector<uint64_t> w(t.words[b]);
+    if (hipMemcpy(w.data(), t.bits[b], w.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return (uint64_t)0;
+    uint64_t n = 0;
+    for (uint64_t x : w) n += (uint64_t)__builtin_popcountll(x);
+    return n;
+  };
+  const uint64_t nseg = host.table.num_segments;
+  printf("\n\033[1;32mGallatin usage stats:\033[1;0m\n");
+  printf("Segment tree sees %llu/%llu segments available\n", (unsigned long long)fill(host.segment_tree),
+         (unsigned long long)nseg);
+  for (uint32_t c = 0; c < kNumTrees; c++)
+    printf("Tree %u: size %llu, owns %llu/%llu\n", c, (unsigned long long)table_type::tree_alloc_size(c),
+           (unsigned long long)fill(host.sub_trees[c]), (unsigned long long)nseg);
+  // per-class live blocks from the segment counters (reference :1976-2011)
+  std::vector<uint32_t> ids(nseg);
+  std::vector<segment_counters> segs(nseg);
+  if (hipMemcpy(ids.data(), host.table.chunk_ids, nseg * 4, hipMemcpyDeviceToHost) != hipSuccess) return;
+  if (hipMemcpy(segs.data(), host.table.segs, nseg * sizeof(segment_counters), hipMemcpyDeviceToHost) != hipSuccess)
+    return;
+  printf("\n\033[1;32mTree usage stats:\033[1;0m\n");
+  for (uint32_t c = 0; c < kNumTrees; c++) {
+    uint64_t count = 0;
+    int64_t avail = 0, in_use = 0;
+    const int64_t nb = (int64_t)table_type::blocks_in_segment(c);
+    for (uint64_t s = 0; s < nseg; s++) {
+      if (ids[s] != c || segs[s].active == -1) continue;
+      count++;
+      avail += segs[s].active;
+      in_use += nb - segs[s].active;
+    }
+    printf("Tree %u: %llu live blocks | avg available %f / %lld | avg in use %f / %lld\n", c,
+           (unsigned long long)count, count ? 1.0 * (double)avail / (double)count : 0.0, (long long)nb,
+           count ? 1.0 * (double)in_use / (double)count : 0.0, (long long)nb);
+  }
+  uint32_t ctl[kCtlWords];
+  if (hipMemcpy(ctl, host.control, sizeof(ctl), hipMemcpyDeviceToHost) == hipSuccess)
+    printf("Allocator errors %u, rejected frees %u\n", ctl[kCtlErrors], ctl[kCtlBadFrees]);
+  const uint64_t overhead = sizeof(my_type) + (uint64_t)veb_tree::layers_for(nseg) * 0 +
+                            nseg * (4 + sizeof(segment_counters)) + nseg * kBlocksPerSegment * (sizeof(Block) + 4) +
+                            total_slots() * 4;
+  printf("Allocator is using %llu bytes of overhead\n", (unsigned long long)overhead);
+}
+
+}  // namespace allocators
+}  // namespace gallatin

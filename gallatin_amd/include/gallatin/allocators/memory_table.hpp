@@ -1,0 +1,159 @@
+// memory_table.hpp — blocks, per-segment counters and the recycle queues.
+//
+// Layout in HBM (one array each, all indexed by plain integers):
+//   chunk_ids[num_segments]       u32: 0xFFFF free, c < T class c,
+//                                 T+1+n first segment of an n-segment run
+//   blocks[num_segments * BPS]    Block {malloc_counter, free_counter}, 8 B
+//   queues[num_segments * BPS]    u32 recycled block id + 1 (0 = empty)
+//   segs[num_segments]            segment_counters, 16 B
+// BPS = SEG / (MIN * 4096) block slots per segment; a segment formatted for
+// class c uses the first SEG / ((MIN << c) * 4096) of them.
+// Same records as the reference's alloc_table (memory_table.cuh:229-1356) and
+// Block (block.cuh:59-398); chunk ids widen to u32 so every update is a
+// native 32-bit atomic, and queue entries are block indices, not pointers.
+#pragma once
+
+#include "device_utils.hpp"
+
+namespace gallatin {
+namespace allocators {
+
+constexpr uint32_t kBlockTreeShift = 20;  // class id lives above bit 20 of malloc_counter
+constexpr uint32_t kBlockCountMask = (1u << kBlockTreeShift) - 1u;
+constexpr uint32_t kSlicesPerBlock = 4096;
+constexpr uint32_t kFreeChunk = 0xFFFFu;
+
+struct Block {
+  uint32_t malloc_counter;  // (class << 20) | slices handed out
+  uint32_t free_counter;    // slices returned (or written off) so far
+};
+
+struct segment_counters {
+  int32_t active;        // block grants left (-1: none / segment not formatted)
+  uint32_t queue_alloc;  // grants so far; >= nb means take from the queue
+  uint32_t queue_free;   // returns so far (queue write position)
+  uint32_t queue_final;  // returns whose queue write is complete
+};
+
+// device error counters (reference traps), kept in the control word array
+enum control_word : uint32_t {
+  kCtlLocks = 0,          // bit c: class c is attaching a segment; bit T: segment claim
+  kCtlErrors = 1,         // internal inconsistencies observed
+  kCtlBadFrees = 2,       // frees of pointers the allocator does not own
+  kCtlBootStatus = 3,     // 0 booting, 1 ok, 2 failed
+  kCtlWords = 16
+};
+
+template <uint64_t SEG, uint64_t MIN>
+struct alloc_table {
+  static constexpr uint64_t blocks_per_segment = SEG / (MIN * 4096);
+  static constexpr uint64_t max_allocations_per_segment = SEG / MIN;
+
+  uint32_t* chunk_ids;
+  Block* blocks;
+  uint32_t* queues;
+  segment_counters* segs;
+  char* memory;
+  uint64_t num_segments;
+
+  __host__ __device__ static constexpr uint64_t tree_alloc_size(uint32_t tree) { return MIN << tree; }
+  __host__ __device__ static constexpr uint64_t blocks_in_segment(uint32_t tree) {
+    return SEG / ((MIN << tree) * 4096);
+  }
+
+#if defined(__HIPCC__)
+  __device__ uint32_t read_tree_id(uint64_t segment) const { return atom::load(&chunk_ids[segment]); }
+  __device__ bool set_tree_id(uint64_t segment, uint32_t id) {
+    return atom::cas(&chunk_ids[segment], kFreeChunk, id) == kFreeChunk;
+  }
+  __device__ bool reset_tree_id(uint64_t segment, uint32_t id) {
+    return atom::cas(&chunk_ids[segment], id, kFreeChunk) == id;
+  }
+
+  // format a free segment for class `tree` (reference memory_table.cuh:746-818)
+  __device__ bool setup_segment(uint64_t segment, uint32_t tree) {
+    if (!set_tree_id(segment, tree)) return false;
+    const uint64_t nb = blocks_in_segment(tree);
+    uint32_t* q = queues + segment * blocks_per_segment;
+    for (uint64_t i = 0; i < nb; i++) atom::store(&q[i], 0u);
+    segment_counters* s = &segs[segment];
+    atom::exch(&s->queue_alloc, 0u);
+    atom::exch(&s->queue_free, 0u);
+    atom::exch(&s->queue_final, 0u);
+    // grants open only after the queue state above is in place
+    atom::fence_release();
+    atom::exch(&s->active, (int32_t)nb - 1);
+    return true;
+  }
+
+  // Grant one block of `segment` to class `tree` (reference
+  // memory_table.cuh:915-993): fresh blocks in order first, then recycled
+  // blocks in the order they were returned. Returns the global block id, or
+  // -1 when the segment has nothing to grant or changed class. `last` is set
+  // when this grant took the segment's final block.
+  __device__ int64_t get_block(uint64_t segment, uint32_t tree, bool& last, uint32_t* control) {
+    last = false;
+    segment_counters* s = &segs[segment];
+    int32_t active_count = atom::sub(&s->active, 1);
+    if (active_count < 0) {
+      atom::add(&s->active, 1);
+      return -1;
+    }
+    if (read_tree_id(segment) != tree) {
+      atom::add(&s->active, 1);
+      return -1;
+    }
+    const uint64_t nb = blocks_in_segment(tree);
+    int32_t queue_pos = (int32_t)atom::add(&s->queue_alloc, 1u);
+    int64_t block;
+    if ((uint64_t)queue_pos < nb) {
+      block = (int64_t)(segment * blocks_per_segment + (uint64_t)queue_pos);
+    } else {
+      uint32_t v = atom::exch(&queues[segment * blocks_per_segment + (uint64_t)queue_pos % nb], 0u);
+      if (v == 0u) {  // the reference traps here; count it instead
+        atom::add(&control[kCtlErrors], 1u);
+        return -1;
+      }
+      block = (int64_t)v - 1;
+    }
+    atom::exch(&blocks[block].malloc_counter, tree << kBlockTreeShift);  // Block::init_malloc
+    if (active_count == 0) last = true;
+    return block;
+  }
+
+  // Enqueue a fully returned block; returns its queue slot (reference
+  // memory_table.cuh:1158-1222). Slot writes complete in reservation order,
+  // which is what lets get_block trust the queue entry it is sent to.
+  __device__ uint32_t reserve_segment_slot(uint64_t block, uint64_t segment, uint64_t nb, uint32_t* control) {
+    segment_counters* s = &segs[segment];
+    uint32_t pos = atom::add(&s->queue_free, 1u);
+    uint32_t live = (uint32_t)(pos % nb);
+    atom::exch(&queues[segment * blocks_per_segment + live], (uint32_t)(block + 1));
+    // Earlier reservations are held by other waves (a wave completes its own
+    // before it can take another), so this wait always ends; the bound only
+    // turns a broken invariant into a counted error instead of a hang.
+    for (uint32_t spin = 0; atom::cas(&s->queue_final, pos, pos + 1) != pos; spin++) {
+      if (spin > (1u << 24)) {
+        atom::add(&control[kCtlErrors], 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    return live;
+  }
+
+  // reference memory_table.cuh:1226-1241; true when this return makes every
+  // block of the segment free and this lane won the right to release it
+  __device__ bool finish_freeing_block(uint64_t segment, uint64_t nb) {
+    atom::fence_release();  // the queue entry is visible before the grant is
+    int32_t return_id = atom::add(&segs[segment].active, 1);
+    if ((int64_t)return_id == (int64_t)nb - 2) {
+      if (atom::cas(&segs[segment].active, (int32_t)nb - 1, (int32_t)-1) == (int32_t)nb - 1) return true;
+    }
+    return false;
+  }
+#endif  // __HIPCC__
+};
+
+}  // namespace allocators
+}  // namespace gallatin

@@ -1,0 +1,260 @@
+// veb.hpp — 64-ary bit-vector tree over segment indices (device side).
+//
+// Same structure and update rules as the reference's veb_tree
+// (include/gallatin/allocators/veb.cuh:109-1228): layer L-1 holds one bit per
+// item, each layer above holds one bit per 64-bit word of the layer below,
+// layer 0 is the top summary. Words are only touched with agent-scope atomics.
+// The search is word-at-a-time: a successor query is at most L dependent
+// 64-bit loads going up and L going down, each resolved with one ffs.
+#pragma once
+
+#include "device_utils.hpp"
+
+namespace gallatin {
+namespace allocators {
+
+constexpr int kMaxVebLayers = 6;           // universes below 2^36 items
+constexpr int kVebHashedAttempts = 15;     // reference VEB_MAX_ATTEMPTS, veb.cuh:59
+constexpr int kSuccessorRestarts = 1 << 16;  // safety bound on stale-summary restarts
+
+struct veb_tree {
+  uint64_t* bits[kMaxVebLayers];    // [0] top summary ... [num_layers-1] items
+  uint64_t universe[kMaxVebLayers]; // items represented by each layer
+  uint64_t words[kMaxVebLayers];    // 64-bit words of each layer
+  uint64_t seed;                    // seeds the hashed start of find_random_valid_index
+  uint64_t total_universe;
+  int num_layers;
+
+  // layers = floor((h-1)/6)+1 for h = bit length of the universe
+  // (reference veb.cuh:601-605)
+  __host__ __device__ static int layers_for(uint64_t universe) {
+    int h = 64 - utils::clz64(universe);
+    return (h - 1) / 6 + 1;
+  }
+
+#if defined(__HIPCC__)
+  __device__ uint64_t load_word(int layer, uint64_t w) const { return atom::load(&bits[layer][w]); }
+
+  // first set bit strictly above `low` in word `high` of `layer` (-1: any)
+  __device__ int find_next(int layer, uint64_t high, int low) const {
+    if (high >= universe[layer]) return -1;
+    uint64_t w = load_word(layer, high);
+    if (low >= 0) w &= ~utils::bitmask((uint64_t)low + 1);
+    return utils::ffs64(w) - 1;
+  }
+
+  __device__ bool query(uint64_t v) const { return (load_word(num_layers - 1, v >> 6) >> (v & 63)) & 1ull; }
+
+  // clear item v; clear summary bits upward while a word empties
+  __device__ bool remove(uint64_t v) {
+    uint64_t high = v >> 6;
+    int low = (int)(v & 63);
+    int layer = num_layers - 1;
+    uint64_t old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
+    if (!(old & (1ull << low))) return false;
+    while (utils::popc64(old) == 1 && layer > 0) {
+      layer--;
+      low = (int)(high & 63);
+      high >>= 6;
+      old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
+    }
+    return true;
+  }
+
+  // set item v; set summary bits upward while a word was empty
+  __device__ bool insert_force_update(uint64_t v) {
+    uint64_t high = v >> 6;
+    int low = (int)(v & 63);
+    int layer = num_layers - 1;
+    uint64_t old = atom::fetch_or(&bits[layer][high], 1ull << low);
+    if (old & (1ull << low)) return false;
+    while (old == 0 && layer > 0) {
+      layer--;
+      low = (int)(high & 63);
+      high >>= 6;
+      old = atom::fetch_or(&bits[layer][high], 1ull << low);
+    }
+    return true;
+  }
+
+  // First item strictly above q reachable through the summaries. A summary
+  // bit whose word below turns out empty is stepped over by floating up again.
+  __device__ uint64_t successor_thorough(uint64_t q) const {
+    uint64_t high = q >> 6;
+    int low = (int)(q & 63);
+    int layer = num_layers - 1;
+    for (int restart = 0; restart < kSuccessorRestarts; restart++) {
+      int f;
+      while ((f = find_next(layer, high, low)) == -1) {
+        if (layer == 0) return utils::kFail;
+        layer--;
+        low = (int)(high & 63);
+        high >>= 6;
+      }
+      while (layer != num_layers - 1) {
+        layer++;
+        high = (high << 6) + (uint64_t)f;
+        f = find_next(layer, high, -1);
+        if (f == -1) break;
+      }
+      if (f != -1) return (high << 6) + (uint64_t)f;
+      // stale summary bit at (layer-1): continue strictly after it
+      if (layer == 0) return utils::kFail;
+      layer--;
+      low = (int)(high & 63);
+      high >>= 6;
+    }
+    return utils::kFail;
+  }
+
+  __device__ uint64_t lock_offset(uint64_t start) {
+    if (query(start) && remove(start)) return start;
+    for (int i = 0; i < kSuccessorRestarts; i++) {
+      start = successor_thorough(start);
+      if (start == utils::kFail) return start;
+      if (remove(start)) return start;
+    }
+    return utils::kFail;
+  }
+
+  // hashed start positions (reference veb.cuh:946-977); tid is the caller's
+  // global thread id
+  __device__ uint64_t find_random_valid_index(uint64_t tid) const {
+    uint64_t hash1 = utils::murmur64a(tid, seed);
+    uint64_t hash2 = utils::murmur64a(tid, hash1);
+    for (int attempts = 0; attempts < kVebHashedAttempts; attempts++) {
+      uint64_t start = (hash1 + (uint64_t)attempts * hash2) % total_universe;
+      if (query(start)) return start;
+      uint64_t index = successor_thorough(start);
+      if (index != utils::kFail) return index;
+    }
+    return successor_thorough(0);
+  }
+
+  // lowest free item, claimed (reference veb.cuh:979-991)
+  __device__ uint64_t malloc_first() {
+    for (int attempts = 0; attempts < kVebHashedAttempts; attempts++) {
+      uint64_t offset = lock_offset(0);
+      if (offset != utils::kFail) return offset;
+    }
+    return utils::kFail;
+  }
+
+  // set `count` bits from bit 0 of word `w` upward (undo of a partial claim)
+  __device__ void rollback(uint64_t w, uint64_t count) {
+    uint64_t* b = bits[num_layers - 1];
+    while (count > 64) {
+      atom::fetch_or(&b[w], ~0ull);
+      count -= 64;
+      w++;
+    }
+    atom::fetch_or(&b[w], utils::bitmask(count));
+  }
+
+  // Claim n contiguous items in the item layer, searching words from the top
+  // down: a run is the trailing ones of a word, extended downward through full
+  // words and the high bits of the next lower word (reference veb.cuh:323-452).
+  // A run may not extend below word 0 and never starts in word 0.
+  __device__ uint64_t gather_bottom(uint64_t n) {
+    uint64_t* b = bits[num_layers - 1];
+    uint64_t w = words[num_layers - 1] - 1;
+    while (w != 0) {
+      uint64_t need = n;
+      int run = utils::trailing_ones(atom::load(&b[w]));
+      if (run == 0) {
+        w--;
+        continue;
+      }
+      if ((uint64_t)run >= need) {
+        uint64_t mask = utils::bitmask(need) << (run - need);
+        uint64_t got = atom::fetch_and(&b[w], ~mask);
+        if ((uint64_t)utils::popc64(got & mask) == need) return w * 64 + (run - need);
+        atom::fetch_or(&b[w], got & mask);
+        continue;
+      }
+      uint64_t low_mask = utils::bitmask(run);
+      uint64_t got = atom::fetch_and(&b[w], ~low_mask);
+      if (utils::popc64(got & low_mask) != run) {
+        atom::fetch_or(&b[w], got & low_mask);
+        continue;
+      }
+      need -= run;
+      bool ok = true;
+      while (need > 64) {
+        if (w == 0) {
+          rollback(w, n - need);
+          return utils::kFail;
+        }
+        w--;
+        got = atom::exch(&b[w], 0ull);
+        if (utils::popc64(got) != 64) {
+          atom::fetch_or(&b[w], got);
+          rollback(w + 1, n - need);
+          ok = false;
+          break;
+        }
+        need -= 64;
+      }
+      if (!ok) continue;
+      if (w == 0) {
+        rollback(w, n - need);
+        return utils::kFail;
+      }
+      w--;
+      uint64_t top_mask = utils::bitmask(need) << (64 - need);
+      got = atom::fetch_and(&b[w], ~top_mask);
+      if ((uint64_t)utils::popc64(got & top_mask) == need) return w * 64 + 64 - need;
+      atom::fetch_or(&b[w], got & top_mask);
+      rollback(w + 1, n - need);
+    }
+    return utils::kFail;
+  }
+
+  // claim a run of n items and clear the summary bits of emptied words
+  __device__ uint64_t gather_multiple(uint64_t n) {
+    uint64_t found = gather_bottom(n);
+    if (found == utils::kFail) return found;
+    uint64_t start = found, count = n;
+    for (int layer = num_layers - 2; layer >= 0; layer--) {
+      uint64_t hi = (start + count - 1) / 64 + 1;
+      uint64_t lo = start / 64;
+      start = lo;
+      count = hi - lo;
+      for (uint64_t w = start; w < start + count; w++) {
+        if (atom::load(&bits[layer + 1][w]) == 0) atom::fetch_and(&bits[layer][w >> 6], ~(1ull << (w & 63)));
+      }
+    }
+    return found;
+  }
+
+  // return a run of n items; a summary bit is set only when the word below
+  // is completely full again (reference veb.cuh:1128-1154, 1198-1218)
+  __device__ void return_multiple(uint64_t start, uint64_t n) {
+    uint64_t* b = bits[num_layers - 1];
+    uint64_t w = start / 64;
+    uint64_t first = start % 64;
+    uint64_t left = n;
+    while (left + first > 64) {
+      uint64_t k = 64 - first;
+      atom::fetch_or(&b[w], utils::bitmask(k) << first);
+      first = 0;
+      w++;
+      left -= k;
+    }
+    atom::fetch_or(&b[w], utils::bitmask(left) << first);
+    uint64_t s = start, count = n;
+    for (int layer = num_layers - 2; layer >= 0; layer--) {
+      uint64_t hi = (s + count - 1) / 64 + 1;
+      uint64_t lo = s / 64;
+      s = lo;
+      count = hi - lo;
+      for (uint64_t x = s; x < s + count; x++) {
+        if (atom::load(&bits[layer + 1][x]) == ~0ull) atom::fetch_or(&bits[layer][x >> 6], 1ull << (x & 63));
+      }
+    }
+  }
+#endif  // __HIPCC__
+};
+
+}  // namespace allocators
+}  // namespace gallatin

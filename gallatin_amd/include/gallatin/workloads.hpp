@@ -1,0 +1,36 @@
+// workloads.hpp — the synthetic request streams the benchmark kernels use.
+//
+// Every random choice is a pure function of (seed, round, index), so a
+// request can be regenerated anywhere: on the device, on the host, alone or
+// inside a mix.
+#pragma once
+
+#include <stdint.h>
+
+#include "allocators/device_utils.hpp"
+
+namespace gallatin {
+namespace workloads {
+
+// salt 0: size draw of thread i in round r; salt 1: free coin of live slot i
+// after round r; salt 2: payload tag of live slot i
+__host__ __device__ inline uint64_t mixed_key(uint64_t seed, uint64_t round, uint64_t i, uint64_t salt) {
+  return utils::splitmix64(seed + utils::splitmix64((round << 32) ^ i ^ (salt << 60)));
+}
+
+// log-uniform power-of-two sizes 16 B .. 4 KiB
+__host__ __device__ inline uint64_t mixed_size(uint64_t seed, uint64_t round, uint64_t i) {
+  return 16ull << (mixed_key(seed, round, i, 0) % 9);
+}
+
+__host__ __device__ inline bool mixed_coin(uint64_t seed, uint64_t round, uint64_t j) {
+  return mixed_key(seed, round, j, 1) & 1ull;
+}
+
+// nonzero payload tag (zero marks free memory in the atomic check)
+__host__ __device__ inline uint64_t payload_tag(uint64_t seed, uint64_t index) {
+  return mixed_key(seed, 0, index, 2) | 1ull;
+}
+
+}  // namespace workloads
+}  // namespace gallatin

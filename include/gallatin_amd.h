@@ -1,0 +1,183 @@
+/*
+ * gallatin_amd.h — C-ABI of the MI355X-native Gallatin device allocator.
+ *
+ * The in-kernel API (Gallatin<SEG,MIN,MAX>::malloc/free, global_malloc/
+ * global_free) is C++ device code in gallatin_amd/include/gallatin/ and is
+ * inlined into user kernels; it cannot cross a C ABI. This header is the host
+ * boundary: construction/teardown, statistics, metadata snapshots, the
+ * deterministic single-lane replay, and the benchmark workloads. Every entry
+ * point takes plain pointers and sizes and returns an int status
+ * (GAL_OK == 0); none of them exits the process.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   gal_create          Gallatin<SEG,MIN,MAX>::generate_on_device
+ *                       include/gallatin/allocators/gallatin.cuh:324-421
+ *   gal_destroy         Gallatin<SEG,MIN,MAX>::free_on_device  gallatin.cuh:643-668
+ *   gal_print_stats     Gallatin::print_info                   gallatin.cuh:1855-1897
+ *   gal_init_global_allocator / gal_free_global_allocator / gal_print_global_stats
+ *                       include/gallatin/allocators/global_allocator.cuh:48-56, 59-70, 86-97
+ *   gal_replay          a kernel in which one lane calls malloc/free in order
+ *                       (what tests/src/gallatin_single_thread.cu:38-93 does),
+ *                       with the CU id (smid) and thread id supplied explicitly
+ *   gal_run_*           the reference's test kernels (tests/src/gallatin_test.cu:58-219,
+ *                       tests/src/global_churn.cu:185-300) as fixed workloads
+ *
+ * Instantiations: templates cannot cross the ABI, so gal_create accepts only
+ * the (segment_size, min_size, max_size) triples listed by gal_supported().
+ */
+#ifndef GALLATIN_AMD_H
+#define GALLATIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define GAL_OK 0
+#define GAL_ERR_INVALID 1     /* bad argument (null handle, zero size, ...) */
+#define GAL_ERR_UNSUPPORTED 2 /* (segment,min,max) triple not instantiated */
+#define GAL_ERR_OOM 3         /* device allocation failed or pool too small to boot */
+#define GAL_ERR_HIP 4         /* a HIP runtime call failed */
+#define GAL_ERR_BUFFER 5      /* caller buffer too small; *needed says how big */
+#define GAL_ERR_STATE 6       /* allocator detected an internal inconsistency */
+
+typedef struct gal_allocator gal_allocator; /* opaque host handle */
+
+/* ---- replay stream ----
+ * One op per entry, executed in order by a single lane.
+ *   GAL_OP_MALLOC: arg = request size in bytes; result = byte offset of the
+ *                  returned pointer from the pool base, or GAL_NULL_OFFSET.
+ *   GAL_OP_FREE:   arg = index (into the same stream) of the malloc op whose
+ *                  pointer is freed; that malloc must have succeeded and not
+ *                  been freed already. result = 0.
+ * slot: stands in for the CU id (the reference's %smid): the pinned-block
+ *       search starts at slot % slots_of_class.
+ * tid:  the thread id fed to the hashed segment search
+ *       (reference veb.cuh:946-956 hashes threadIdx.x + blockIdx.x*blockDim.x).
+ */
+#define GAL_OP_MALLOC 0u
+#define GAL_OP_FREE 1u
+#define GAL_NULL_OFFSET (~(uint64_t)0)
+
+typedef struct gal_op {
+  uint32_t kind;
+  uint32_t slot;
+  uint64_t arg;
+  uint64_t tid;
+} gal_op;
+
+/* ---- allocator description ---- */
+typedef struct gal_info_t {
+  uint64_t segment_size;       /* bytes per segment */
+  uint64_t min_size;           /* smallest slice size */
+  uint64_t max_size;           /* largest slice size */
+  uint64_t num_segments;       /* pool_bytes / segment_size */
+  uint64_t blocks_per_segment; /* segment_size / (min_size * 4096) */
+  uint64_t num_trees;          /* log2(max) - log2(min) + 1 size classes */
+  uint64_t veb_layers;         /* layers of each vEB tree over num_segments */
+  uint64_t total_slots;        /* pinned-block slots over all classes */
+  uint64_t pool_device_ptr;    /* device address of the pool base */
+  uint64_t device_handle;      /* device address of the Gallatin object */
+  uint64_t snapshot_bytes;     /* size of a gal_snapshot image */
+  uint64_t metadata_bytes;     /* device bytes held by metadata (not pool) */
+} gal_info_t;
+
+/*
+ * Snapshot image (identical layout from the GPU and from the CPU oracle):
+ *   u64 header[8] = {num_segments, blocks_per_segment, num_trees, veb_layers,
+ *                    total_slots, locks, 0, 0}
+ *   segment tree words, layer 0 (top summary) .. layer L-1 (one bit per segment)
+ *   for each class 0..T-1: that class's sub-tree words, same layer order
+ *   u16 chunk_ids[num_segments], zero-padded to a multiple of 8 bytes
+ *       (0xFFFF free; c < T owned by class c; T+1+n head of an n-segment run)
+ *   u32 {malloc_counter, free_counter} per block, num_segments*blocks_per_segment
+ *   u32 queue[num_segments*blocks_per_segment]  (0 empty, else block id + 1)
+ *   u32 {active, queue_alloc, queue_free, queue_final} per segment (active as i32)
+ *   u32 slots[total_slots], class-major (0 empty, else block id + 1)
+ *   zero-padded to a multiple of 8 bytes
+ */
+
+/* (segment_size, min_size, max_size) triples compiled into the library.
+ * Writes up to cap triples (3 u64 each) into out; returns the count. */
+int gal_supported(uint64_t* out, int cap);
+
+int gal_create(uint64_t segment_size, uint64_t min_size, uint64_t max_size,
+               uint64_t pool_bytes, uint64_t seed, gal_allocator** out);
+int gal_destroy(gal_allocator* h);
+int gal_get_info(gal_allocator* h, gal_info_t* info);
+int gal_print_stats(gal_allocator* h);
+int gal_snapshot(gal_allocator* h, void* buf, uint64_t cap, uint64_t* needed);
+
+/* Single-lane deterministic replay; results[i] per op (see above). */
+int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* results);
+
+/* Device error counters: internal inconsistencies (where the reference would
+ * trap) and rejected frees of pointers the allocator does not own. */
+int gal_errors(gal_allocator* h, uint64_t* internal_errors, uint64_t* bad_frees);
+
+/* Global allocator (Gallatin<16 MiB, 16, 4096> singleton).
+ * Replaces init_global_allocator / free_global_allocator / print_global_stats,
+ * reference global_allocator.cuh:48-56, 59-70, 86-97. */
+int gal_init_global_allocator(uint64_t num_bytes, uint64_t seed, int print_info);
+int gal_free_global_allocator(void);
+int gal_print_global_stats(void);
+/* one kernel of n threads: global_malloc(size), tag, verify, global_free */
+int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors);
+
+/* ---- workloads ----
+ * Kernel durations come from HIP events recorded on the library's own stream
+ * around each launch (the stream the kernels run on). */
+typedef struct gal_run_result {
+  uint64_t mallocs;        /* successful mallocs */
+  uint64_t frees;          /* frees issued */
+  uint64_t misses;         /* mallocs that returned nullptr */
+  uint64_t tag_errors;     /* payload tag checks that failed (overlap) */
+  uint64_t alloc_errors;   /* growth of the allocator's internal error count */
+  uint64_t launches;       /* kernels launched */
+  double malloc_kernel_ms; /* summed durations of the kernels that malloc */
+  double free_kernel_ms;   /* summed durations of the kernels that only free */
+  double total_ms;         /* first launch start to last launch end */
+} gal_run_result;
+
+/* Tag check modes for the workloads. */
+#define GAL_VERIFY_STORE 0u  /* store the tag, load it back */
+#define GAL_VERIFY_ATOMIC 1u /* atomic exchange in (expects 0) and out (expects tag): detects sharing */
+
+/* One kernel of n_threads threads. Each thread: p = malloc(size), write a
+ * tag derived from (seed, thread), check it, free(p). The device API's
+ * in-kernel interface of the reference test kernels
+ * (tests/src/gallatin_test.cu:58-110, tests/src/global_churn.cu:185-245). */
+int gal_run_plain(gal_allocator* h, uint64_t n_threads, uint64_t size, uint64_t seed, uint32_t verify,
+                  gal_run_result* out);
+
+/* Mixed-size rounds. Round r: thread i mallocs 16 << (k mod 9) bytes with k
+ * drawn from (seed, r, i), stores the pointer in the live table and tags the
+ * allocation; then every live pointer j is freed (after its tag is checked)
+ * when the coin drawn from (seed, r, j) is odd. gal_mixed_reset empties the
+ * live table (allocations still live are freed). */
+int gal_mixed_reset(gal_allocator* h, uint64_t threads_per_round, uint32_t rounds, uint64_t seed);
+int gal_mixed_round(gal_allocator* h, uint32_t round, uint32_t verify, gal_run_result* out);
+int gal_mixed_drain(gal_allocator* h, uint32_t verify, gal_run_result* out);
+/* live allocations of the table: byte offsets from the pool base, request
+ * sizes, and live-table slots (round * threads_per_round + thread) */
+int gal_mixed_live(gal_allocator* h, uint64_t* offsets, uint64_t* sizes, uint64_t* slots, uint64_t cap,
+                   uint64_t* count);
+/* reset + all rounds + drain; one benchmark step */
+int gal_run_mixed(gal_allocator* h, uint64_t threads_per_round, uint32_t rounds, uint64_t seed, uint32_t verify,
+                  gal_run_result* out);
+
+/* Malloc latency samples of the last mixed run: one per wavefront (its lane
+ * 0), the wave's time inside malloc in device wall-clock ticks; tick_ns is
+ * the tick length. ticks == NULL just reports the count. */
+int gal_latency_samples(gal_allocator* h, uint32_t* ticks, uint64_t cap, uint64_t* count, double* tick_ns);
+
+/* Copy n bytes of the pool starting at byte offset off to host memory. */
+int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALLATIN_AMD_H */

@@ -1,0 +1,105 @@
+"""Allocator invariants, shared by the oracle tests and the GPU tests.
+
+Live-set checks (any time): no two live allocations overlap, each starts on
+its granted size's alignment, each lies inside the pool.
+Quiescent checks (no allocation live, no call in flight) on a snapshot:
+  - a segment is free (chunk id 0xFFFF) exactly when its segment-tree bit is set
+  - a class tree only lists segments formatted for that class
+  - per formatted segment: active == nb - 1 - queue_alloc + queue_free,
+    queue_final == queue_free, and the blocks granted and not returned are
+    exactly the pinned blocks that live in it
+  - a pinned block has returned every slice it handed out
+    (free_counter == slices claimed) and carries its class in malloc_counter;
+    every other block has free_counter == 0
+  - no lock bit is held
+"""
+import numpy as np
+
+from gallatin_amd import snapshot as snap
+
+NULL = (1 << 64) - 1
+
+
+def granted_bytes(seg, mn, mx, size, route):
+    path, tree, count = route(seg, mn, mx, size)
+    if path == 0:
+        return (mn << tree) * count, (mn << tree) if count == 1 else mx
+    if path == 1:
+        b = (mn << tree) * 4096
+        return b, b
+    return count * seg, seg
+
+
+def check_live(offsets, sizes, pool_bytes, seg, mn, mx, route):
+    """offsets/sizes: numpy arrays of live allocations (byte offsets)."""
+    offsets = np.asarray(offsets, dtype=np.uint64)
+    sizes = np.asarray(sizes, dtype=np.uint64)
+    if len(offsets) == 0:
+        return
+    cache = {}
+    granted = np.zeros(len(sizes), dtype=np.uint64)
+    align = np.zeros(len(sizes), dtype=np.uint64)
+    for i, s in enumerate(sizes.tolist()):
+        if s not in cache:
+            cache[s] = granted_bytes(seg, mn, mx, s, route)
+        granted[i], align[i] = cache[s]
+    assert np.all(offsets % align == 0), "misaligned allocation"
+    assert np.all(offsets + granted <= np.uint64(pool_bytes)), "allocation outside the pool"
+    order = np.argsort(offsets, kind="stable")
+    o, g = offsets[order], granted[order]
+    overlap = np.nonzero(o[:-1] + g[:-1] > o[1:])[0]
+    assert len(overlap) == 0, f"{len(overlap)} overlapping allocations, first at offset {int(o[overlap[0]])}"
+
+
+def bits_of(words):
+    out = []
+    for w, val in enumerate(np.asarray(words, dtype=np.uint64).tolist()):
+        while val:
+            low = val & -val
+            out.append(w * 64 + low.bit_length() - 1)
+            val ^= low
+    return out
+
+
+def check_quiescent(image, seg_bytes, min_size):
+    s = snap.parse(image)
+    nseg, bps, T = s["num_segments"], s["blocks_per_segment"], s["num_trees"]
+    chunk = s["chunk_ids"].astype(np.int64)
+    assert s["locks"] == 0, "a lock bit is held"
+
+    free_bits = np.zeros(nseg, dtype=bool)
+    free_bits[bits_of(s["segment_tree"][-1])] = True
+    assert np.array_equal(free_bits, chunk == 0xFFFF), "segment tree and chunk ids disagree"
+
+    for c in range(T):
+        listed = bits_of(s["sub_trees"][c][-1])
+        assert all(chunk[x] == c for x in listed), f"class {c} tree lists a segment of another class"
+
+    slots = s["slots"].astype(np.int64)
+    pinned = slots[slots > 0] - 1
+    assert len(np.unique(pinned)) == len(pinned), "a block is pinned twice"
+    pinned_per_seg = np.bincount(pinned // bps, minlength=nseg)
+
+    active = s["active"].astype(np.int64)
+    qa, qf, qfin = (s[k].astype(np.int64) for k in ("queue_alloc", "queue_free", "queue_final"))
+    for x in range(nseg):
+        c = chunk[x]
+        if c < T:
+            nb = seg_bytes // ((min_size << c) * 4096)
+            assert active[x] == nb - 1 - qa[x] + qf[x], f"segment {x}: active count off"
+            assert qfin[x] == qf[x], f"segment {x}: unfinished queue write"
+            outstanding = nb - 1 - active[x]
+            assert outstanding == pinned_per_seg[x], f"segment {x}: {outstanding} blocks out, {pinned_per_seg[x]} pinned"
+        elif c == 0xFFFF:
+            assert active[x] == -1, f"free segment {x} has active {active[x]}"
+            assert pinned_per_seg[x] == 0
+
+    mc, fc = s["malloc_counter"].astype(np.int64), s["free_counter"].astype(np.int64)
+    is_pinned = np.zeros(nseg * bps, dtype=bool)
+    is_pinned[pinned] = True
+    count = mc & ((1 << 20) - 1)
+    assert np.all(fc[~is_pinned] == 0), "a block that is not pinned still counts frees"
+    assert np.all(fc[is_pinned] == count[is_pinned]), "a pinned block has slices out"
+    seg_of = pinned // bps
+    assert np.all((mc[pinned] >> 20) == chunk[seg_of]), "pinned block of the wrong class"
+    return s

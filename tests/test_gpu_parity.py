@@ -1,0 +1,193 @@
+"""GPU parity: the HIP allocator against the CPU oracle, through the C-ABI.
+
+Single-lane replay: the GPU must return bit-identical byte offsets AND leave
+bit-identical metadata (every vEB word, chunk id, block counter, queue entry,
+segment counter and pinned slot) after the same op stream.
+Concurrent runs: exact invariants (no overlap, alignment, in-pool, no tag
+clash, no internal error, quiescent metadata consistent after full drain).
+"""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import gallatin_amd as ga
+from gallatin_amd import snapshot as snap
+from oracle import oracle
+from tests import invariants
+
+pytestmark = pytest.mark.gpu
+
+NULL = (1 << 64) - 1
+MiB = 1 << 20
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def valid_stream(o, ops):
+    """Run ops on the oracle and keep only frees of mallocs that succeeded."""
+    out, remap, res = [], {}, []
+    for i, (kind, arg, tid, slot) in enumerate(ops):
+        if kind == 0:
+            remap[i] = len(out)
+            out.append((0, arg, tid, slot))
+            res.append(o.malloc(arg, tid, slot))
+        else:
+            j = remap[arg]
+            if res[j] == NULL:
+                continue
+            assert o.free(res[j]) == 0
+            out.append((1, j, tid, slot))
+            res.append(0)
+    return out, np.array(res, dtype=np.uint64)
+
+
+def random_ops(seed, n, sizes, free_p=0.45, slots=256):
+    rng = random.Random(seed)
+    live, ops = [], []
+    for _ in range(n):
+        if live and rng.random() < free_p:
+            ops.append((1, live.pop(rng.randrange(len(live))), 0, 0))
+        else:
+            ops.append((0, rng.choice(sizes), rng.randrange(1 << 24), rng.randrange(slots)))
+            live.append(len(ops) - 1)
+    return ops
+
+
+CASES = [
+    # name, pool, (seg, min, max), sizes, n_ops, free_p
+    ("plain16", 1 << 30, (16 * MiB, 16, 4096), [16], 9000, 0.3),
+    ("mixed_slices", 1 << 30, (16 * MiB, 16, 4096), [16, 32, 48, 64, 256, 1000, 2048, 4096], 8000, 0.45),
+    ("big_slices", 1 << 30, (16 * MiB, 16, 4096), [16, 4097, 8192, 12000, 16384, 32768], 6000, 0.45),
+    ("blocks_segments", 4 << 30, (16 * MiB, 16, 4096), [16, 40000, 64 * 1024, MiB, 16 * MiB, 17 * MiB, 40 * MiB],
+     3000, 0.45),
+    ("exhaustion", 64 * MiB, (MiB, 16, 256), [16, 32, 256, 1024, 8192, 65536, 2 * MiB], 8000, 0.35),
+    ("min128", 1 << 30, (4 * MiB, 128, 1024), [16, 100, 128, 512, 1024, 3000, 200000], 6000, 0.45),
+]
+
+
+@pytest.mark.parametrize("name,pool,cfg,sizes,n,free_p", CASES, ids=[c[0] for c in CASES])
+def test_replay_bit_exact(name, pool, cfg, sizes, n, free_p):
+    seg, mn, mx = cfg
+    seed = 1000 + len(name)
+    say(f"[replay] case={name} pool={pool} cfg={cfg} ops={n}")
+    o = oracle.Oracle(pool, seed, segment=seg, min_size=mn, max_size=mx)
+    g = ga.Gallatin.generate_on_device(pool, seed, segment_size=seg, min_size=mn, max_size=mx)
+    try:
+        # construction (the booted pinned blocks) is already bit-identical
+        boot_g, boot_o = g.snapshot(), o.snapshot()
+        assert snap.first_difference(boot_g, boot_o) is None
+        assert np.array_equal(boot_g, boot_o)
+
+        ops, expect = valid_stream(o, random_ops(seed, n, sizes, free_p))
+        got = g.replay(ops)
+        diff = np.nonzero(got != expect)[0]
+        assert len(diff) == 0, f"first mismatch at op {diff[0]}: {ops[diff[0]]} gpu={got[diff[0]]} cpu={expect[diff[0]]}"
+        sg, so = g.snapshot(), o.snapshot()
+        assert snap.first_difference(sg, so) is None
+        assert np.array_equal(sg, so)
+        assert g.errors() == (0, 0)
+        assert o.faults() == 0
+    finally:
+        g.free_on_device()
+
+
+def test_replay_free_all_restores_quiescent_state():
+    pool, seed = 1 << 30, 7
+    o = oracle.Oracle(pool, seed)
+    g = ga.Gallatin.generate_on_device(pool, seed)
+    try:
+        ops = [(0, 16 << (i % 9), i, i % 200) for i in range(20000)]
+        ops += [(1, i, 0, 0) for i in range(20000)]
+        ops, expect = valid_stream(o, ops)
+        got = g.replay(ops)
+        assert np.array_equal(got, expect)
+        sg = g.snapshot()
+        assert np.array_equal(sg, o.snapshot())
+        invariants.check_quiescent(sg, 16 * MiB, 16)
+    finally:
+        g.free_on_device()
+
+
+@pytest.mark.parametrize("threads", [1 << 16, 1 << 20])
+def test_plain_kernel_concurrent(threads):
+    say(f"[plain] threads={threads}")
+    g = ga.Gallatin.generate_on_device(1 << 30, 111)
+    try:
+        r = g.run_plain(threads, 16, 5, verify=ga.GAL_VERIFY_ATOMIC)
+        assert r["misses"] == 0 and r["tag_errors"] == 0 and r["alloc_errors"] == 0
+        assert r["mallocs"] == threads and r["frees"] == threads
+        invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+        # again, on the state the first run left
+        r = g.run_plain(threads, 16, 6, verify=ga.GAL_VERIFY_ATOMIC)
+        assert r["misses"] == 0 and r["tag_errors"] == 0 and r["alloc_errors"] == 0
+        invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+        assert g.errors() == (0, 0)
+    finally:
+        g.free_on_device()
+
+
+@pytest.mark.parametrize("size", [24, 100, 4096, 8192, 32768, 1 << 20])
+def test_plain_kernel_sizes(size):
+    say(f"[plain-size] size={size}")
+    g = ga.Gallatin.generate_on_device(4 << 30, 3)
+    try:
+        threads = 1 << 14 if size >= 1 << 20 else 1 << 18
+        r = g.run_plain(threads, size, 9, verify=ga.GAL_VERIFY_ATOMIC)
+        assert r["tag_errors"] == 0 and r["alloc_errors"] == 0
+        assert r["mallocs"] + r["misses"] == threads
+        if size <= 32768:
+            assert r["misses"] == 0
+        invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+    finally:
+        g.free_on_device()
+
+
+def test_mixed_rounds_invariants():
+    pool, threads, rounds = 2 << 30, 1 << 18, 6
+    say(f"[mixed] pool={pool} threads={threads} rounds={rounds}")
+    g = ga.Gallatin.generate_on_device(pool, 21)
+    try:
+        g.mixed_reset(threads, rounds, 77)
+        for r in range(rounds):
+            res = g.mixed_round(r, verify=ga.GAL_VERIFY_ATOMIC)
+            assert res["tag_errors"] == 0 and res["alloc_errors"] == 0, res
+            offs, sizes, _ = g.mixed_live()
+            invariants.check_live(offs, sizes, pool, 16 * MiB, 16, 4096, oracle.route)
+        res = g.mixed_drain(verify=ga.GAL_VERIFY_ATOMIC)
+        assert res["tag_errors"] == 0 and res["alloc_errors"] == 0
+        invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+        lat = g.latency_ns()
+        assert len(lat) > 0 and np.all(lat >= 0)
+        assert g.errors() == (0, 0)
+    finally:
+        g.free_on_device()
+
+
+def test_mixed_full_workload_repeatable():
+    pool, threads, rounds = 8 << 30, 1 << 20, 4
+    say(f"[mixed-full] pool={pool} threads={threads} rounds={rounds}")
+    g = ga.Gallatin.generate_on_device(pool, 2)
+    try:
+        for step in range(2):
+            res = g.run_mixed(threads, rounds, 100 + step, verify=ga.GAL_VERIFY_ATOMIC)
+            assert res["tag_errors"] == 0 and res["alloc_errors"] == 0, res
+            assert res["frees"] == res["mallocs"]
+            invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+    finally:
+        g.free_on_device()
+
+
+def test_global_allocator():
+    say("[global]")
+    ga.init_global_allocator(1 << 30, 111, print_info=False)
+    try:
+        misses, bad = ga.global_smoke(1 << 18, 64)
+        assert misses == 0 and bad == 0
+        ga.print_global_stats()
+    finally:
+        ga.free_global_allocator()
