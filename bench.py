@@ -86,14 +86,27 @@ def main():
     log(f"[bench] creating Gallatin<16MiB,16,4096> pool={pool} threads/round={threads} rounds={rounds}")
     g = ga.Gallatin.generate_on_device(pool, args.seed)
 
+    def step(seed, tag):
+        """One step: reset, every round (malloc kernel + free kernel), drain."""
+        g.mixed_reset(threads, rounds, seed)
+        total = None
+        for r in range(rounds):
+            res = g.mixed_round(r)
+            log(f"[bench] {tag} round {r}: mallocs {res['mallocs']} frees {res['frees']} misses {res['misses']} "
+                f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f}")
+            total = res if total is None else {k: total[k] + res[k] for k in total}
+        res = g.mixed_drain()
+        total = {k: total[k] + res[k] for k in total}
+        return total
+
     for w in range(args.warmup):
-        r = g.run_mixed(threads, rounds, args.seed + 1000 + w)
+        r = step(args.seed + 1000 + w, f"warmup {w}")
         log(f"[bench] warmup {w}: {r}")
 
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        results.append(g.run_mixed(threads, rounds, args.seed + k))
+        results.append(step(args.seed + k, f"step {k}"))
     t1 = time.perf_counter()
     for k, r in enumerate(results):
         log(f"[bench] step {k}: {r}")

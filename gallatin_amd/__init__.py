@@ -104,6 +104,28 @@ class Gallatin:
         _capi.check("gal_errors", _lib.gal_errors(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    ERROR_SITES = (
+        "queue_empty",
+        "reset_count",
+        "chunk_reset",
+        "setup",
+        "queue_spin",
+        "final_spin",
+        "fill_class",
+        "run_reset",
+        "fill_count",
+        "init_counter",
+        "grant_class",
+        "claim_class",
+        "free_class",
+        "over_return",
+    )
+
+    def error_sites(self):
+        buf = (ctypes.c_uint32 * 14)()
+        _capi.check("gal_error_sites", _lib.gal_error_sites(self._h, buf, 14))
+        return {name: buf[i] for i, name in enumerate(self.ERROR_SITES) if buf[i]}
+
     def snapshot(self):
         need = ctypes.c_uint64()
         _lib.gal_snapshot(self._h, None, 0, ctypes.byref(need))

@@ -142,7 +142,15 @@ namespace wave {
 // wave64 helpers. "group" masks are 64-bit lane masks over the wavefront.
 __device__ inline uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-__device__ inline uint64_t active_mask() { return __builtin_amdgcn_ballot_w64(true); }
+// The exec mask at this point of the program. A ballot of `true` has no
+// operands, so the optimizer may evaluate it earlier (outside the branch or
+// loop that narrowed the mask); a volatile read of exec stays where it is.
+// Every ballot taken inside a narrowed region is ANDed with such a mask.
+__device__ inline uint64_t active_mask() {
+  uint64_t m;
+  asm volatile("s_mov_b64 %0, exec" : "=s"(m));
+  return m;
+}
 // lanes strictly below this one
 __device__ inline uint64_t lanemask_lt() {
   uint32_t l = lane_id();
@@ -151,13 +159,6 @@ __device__ inline uint64_t lanemask_lt() {
 __device__ inline uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
 __device__ inline int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
 __device__ inline int last_lane(uint64_t m) { return 63 - __clzll((long long)m); }
-__device__ inline uint32_t readfirst(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline int32_t readfirst_i(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline uint64_t readfirst64(uint64_t v) {
-  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
 __device__ inline uint32_t readlane(uint32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ inline uint64_t readlane64(uint64_t v, int lane) {
   uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);

@@ -37,6 +37,7 @@ namespace allocators {
 constexpr int kMallocLoopAttempts = 10;     // GALLATIN_MALLOC_LOOP_ATTEMPTS
 constexpr int kSliceAttempts = 500;         // GALLATIN_MAX_ATTEMPTS
 constexpr int kRequestBlockAttempts = 1000; // REQUEST_BLOCK_MAX_ATTEMPTS
+constexpr int kRequestBlockPasses = 1 << 20; // safety bound on lost-race retries
 constexpr int kRequestBlockOomAttempts = 2; // failures with no segment anywhere
 constexpr int kBlockAttempts = 4;           // GALLATIN_MALLOC_BLOCK_ATTEMPTS (500) on OOM
 constexpr int kSegmentAttempts = 4;         // GALLATIN_MALLOC_SEGMENT_ATTEMPTS (500) on OOM
@@ -100,7 +101,7 @@ struct Gallatin {
     return (atom::fetch_or(lock_word(), m) & m) == 0;
   }
   __device__ void release_tree_lock(uint32_t tree) { atom::fetch_and(lock_word(), ~(1u << tree)); }
-  __device__ void count_error() { atom::add(&control[kCtlErrors], 1u); }
+  __device__ void count_error(uint32_t site) { count_error_at(control, site); }
 
   __device__ uint32_t* slot_ptr(uint32_t tree, uint32_t index) const { return &slots[slot_offset[tree] + index]; }
 
@@ -123,7 +124,7 @@ struct Gallatin {
     uint64_t segment = segment_tree.malloc_first();
     if (segment == utils::kFail) return -1;
     if (!table.setup_segment(segment, tree)) {
-      count_error();
+      count_error(kErrSetup);
       return -1;
     }
     atom::fence();
@@ -136,19 +137,31 @@ struct Gallatin {
   // none with free blocks (reference :1525-1631); tid seeds the hashed search
   __device__ __noinline__ int64_t request_new_block_from_tree(uint32_t tree, uint64_t tid) {
     int oom = 0;
-    for (int attempts = 0; attempts < kRequestBlockAttempts; attempts++) {
+    // Like the reference, only passes that fail for lack of memory count
+    // toward giving up; passes that lose a race (the class lock is held, the
+    // segment found is exhausted) retry, up to a large safety bound.
+    int attempts = 0;
+    for (int pass = 0; attempts < kRequestBlockAttempts && pass < kRequestBlockPasses; pass++) {
       uint64_t segment = sub_trees[tree].find_random_valid_index(tid);
       if (segment == utils::kFail) {
+        // out of memory is confirmed without the class lock: gather_new_segment
+        // would fail the same way (its malloc_first sees exactly this)
+        if (segment_tree.nothing_to_claim()) {
+          attempts++;
+          if (++oom >= kRequestBlockOomAttempts) return -1;
+          continue;
+        }
         if (acquire_tree_lock(tree)) {
           int64_t got = gather_new_segment(tree);
           release_tree_lock(tree);
           if (got == -1) {
+            attempts++;
             if (++oom >= kRequestBlockOomAttempts) return -1;
             continue;
           }
           segment = (uint64_t)got;
         } else {
-          wave::backoff(attempts);  // another lane is attaching a segment
+          wave::backoff(pass);  // another lane is attaching a segment
           continue;
         }
       }
@@ -170,6 +183,9 @@ struct Gallatin {
 
   // return a fully freed block to its segment (reference :1637-1726)
   __device__ __noinline__ void return_block(uint64_t block, uint64_t segment, uint32_t tree) {
+    // everything this wave did to the block (its counter reset above all) is
+    // performed before the block becomes grantable again
+    atom::fence_release();
     const uint64_t nb = table_type::blocks_in_segment(tree);
     const uint32_t slot = table.reserve_segment_slot(block, segment, nb, control);
     // 1.0*slot/nb >= 0.1 && 1.0*(slot-1)/nb < 0.1 for power-of-two nb
@@ -178,12 +194,12 @@ struct Gallatin {
       sub_trees[tree].insert_force_update(segment);
       atom::fence();
     }
-    const bool release = table.finish_freeing_block(segment, nb);
+    const bool release = table.finish_freeing_block(segment, nb, control);
     atom::fence();
     if (release) {
       sub_trees[tree].remove(segment);
       atom::fence();
-      if (!table.reset_tree_id(segment, tree)) count_error();
+      if (!table.reset_tree_id(segment, tree)) count_error(kErrChunkReset);
       atom::fence();
       segment_tree.insert_force_update(segment);
     }
@@ -198,11 +214,20 @@ struct Gallatin {
   // recycles the block (reference block.cuh:87-98, gallatin.cuh:1790-1799)
   __device__ void free_slices(uint64_t block, uint32_t n) {
     Block* b = &table.blocks[block];
+    if ((atom::load(&b->malloc_counter) >> kBlockTreeShift) != table.read_tree_id(block / kBlocksPerSegment))
+      count_error(kErrFreeClass);
     uint32_t old = atom::add_release(&b->free_counter, n);
-    if (old + n == kSlicesPerBlock) {
-      atom::exch(&b->free_counter, 0u);
-      free_block(block);
-    }
+    if (old + n == kSlicesPerBlock) recycle_block(block);
+  }
+
+  // reset the free count of a block whose 4096 slices are all back and hand
+  // the block to its segment (reference block.cuh:200-212 + free_block)
+  __device__ void recycle_block(uint64_t block) {
+    // the reset's old value is read: the reset must be performed before the
+    // block is queued, or a late reset would wipe its next owner's count
+    const uint32_t prev = atom::exch(&table.blocks[block].free_counter, 0u);
+    if (prev != kSlicesPerBlock) count_error(kErrResetCount);
+    free_block(block);
   }
 
   // swap a spent pinned block for a fresh one (reference :1083-1138); a
@@ -213,6 +238,7 @@ struct Gallatin {
     atom::fence();
     int64_t fresh = request_new_block_from_tree(tree, tid);
     if (fresh < 0) return false;
+    atom::fence_release();  // the block's counter reset precedes its publication
     if (atom::cas(s, 0u, (uint32_t)(fresh + 1)) != 0u) {
       free_block((uint64_t)fresh);
       return false;
@@ -224,6 +250,7 @@ struct Gallatin {
   __device__ __noinline__ bool refill_slot(uint32_t tree, uint32_t index, uint64_t tid) {
     int64_t fresh = request_new_block_from_tree(tree, tid);
     if (fresh < 0) return false;
+    atom::fence_release();
     if (atom::cas(slot_ptr(tree, index), 0u, (uint32_t)(fresh + 1)) != 0u) free_block((uint64_t)fresh);
     return true;
   }
@@ -231,21 +258,21 @@ struct Gallatin {
   // ------------------------------------------------------- slice claims
   // One claim for the lanes of the current exec mask (all of class `tree`).
   // Sets done/result for the lanes that are finished.
-  __device__ void claim_group(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint, uint64_t& result,
-                              bool& done) {
-    const uint64_t group = wave::active_mask();
+  __device__ void claim_group(uint64_t group, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
+                              uint64_t& result, bool& done, bool& oom) {
     const int leader = wave::first_lane(group);
     const bool is_leader = (int)wave::lane_id() == leader;
-    const uint32_t lead_hint = wave::readfirst(hint);
-    const uint64_t lead_tid = wave::readfirst64(tid);
+    const uint32_t lead_hint = wave::readlane(hint, leader);
+    const uint64_t lead_tid = wave::readlane64(tid, leader);
 
     const uint32_t index = valid_block_index(tree, lead_hint);
-    const uint32_t entry = wave::readfirst(atom::load(slot_ptr(tree, index)));
+    const uint32_t entry = wave::readlane(atom::load(slot_ptr(tree, index)), leader);
     if (entry == 0u) {
       bool ok = false;
       if (is_leader) ok = refill_slot(tree, index, lead_tid);
-      if (wave::ballot(ok) == 0) {  // class out of memory
+      if ((wave::ballot(ok) & group) == 0) {  // class out of memory
         done = true;
+        oom = true;
         result = utils::kFail;
       }
       return;
@@ -255,7 +282,7 @@ struct Gallatin {
 
     // exclusive scan of the slice counts in lane order
     uint32_t prefix, total;
-    if (wave::ballot(count != 1u) == 0) {
+    if ((wave::ballot(count != 1u) & group) == 0) {
       prefix = wave::popc(group & wave::lanemask_lt());
       total = wave::popc(group);
     } else {
@@ -277,22 +304,20 @@ struct Gallatin {
     // straddling request's tail) are written off as already freed
     // (reference block.cuh:267-281): in-range slices minus successful lanes.
     const uint32_t in_range = c0 >= kSlicesPerBlock ? 0u : ((c0 + total < kSlicesPerBlock ? c0 + total : kSlicesPerBlock) - c0);
-    const uint32_t written_off = in_range - wave::popc(wave::ballot(valid));
+    const uint64_t valid_lanes = wave::ballot(valid) & group;
+    const uint32_t written_off = in_range - wave::popc(valid_lanes);
     bool completed = false;
     if (written_off > 0 && is_leader) {
       uint32_t old = atom::add(&b->free_counter, written_off);
       completed = (old + written_off == kSlicesPerBlock);
     }
-    completed = wave::ballot(completed) != 0;
-    if (wave::ballot(should_replace) != 0 && is_leader) replace_block(tree, index, block, lead_tid);
-    if (completed && is_leader) {  // the write-off finished the block: recycle it
-      atom::exch(&b->free_counter, 0u);
-      free_block(block);
-    }
+    completed = (wave::ballot(completed) & group) != 0;
+    if ((wave::ballot(should_replace) & group) != 0 && is_leader) replace_block(tree, index, block, lead_tid);
+    if (completed && is_leader) recycle_block(block);  // the write-off finished the block
     if ((merged >> kBlockTreeShift) != tree) {
       // a stale pinned read: the block now serves another class; give back
       // what the valid lanes drew (reference gallatin.cuh:1037-1056)
-      const uint32_t drawn = wave::popc(wave::ballot(valid));
+      const uint32_t drawn = wave::popc(valid_lanes);
       if (drawn > 0 && is_leader) free_slices(block, drawn);
       return;
     }
@@ -300,21 +325,30 @@ struct Gallatin {
       result = allocation + block * kSlicesPerBlock;
       done = true;
     }
+    if (is_leader && (valid_lanes != 0) && table.read_tree_id(block / kBlocksPerSegment) != tree)
+      count_error(kErrClaimClass);
   }
 
-  // reference gallatin.cuh:865-1078 for every lane of the wave that calls it
-  __device__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint) {
+  // reference gallatin.cuh:865-1078 for every lane of the wave that calls it.
+  // The loops are uniform (driven by lane masks, never by a lane leaving the
+  // loop): the AMDGPU structurizer runs a loop's exit block once, after the
+  // loop, for all lanes that left, so per-group work must not sit in one.
+  // `oom` is set when the failure is a confirmed out-of-memory, which a
+  // further call would only repeat.
+  __device__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
+                                              bool& oom) {
     uint64_t result = utils::kFail;
     bool done = false;
-    for (int attempt = 0; attempt < kSliceAttempts; attempt++) {
-      for (;;) {  // one claim per distinct class among the lanes still looking
-        const uint32_t lead_tree = wave::readfirst(tree);
-        if (tree == lead_tree) {
-          claim_group(lead_tree, count, tid, hint, result, done);
-          break;
-        }
+    oom = false;
+    uint64_t looking = wave::active_mask();
+    for (int attempt = 0; attempt < kSliceAttempts && looking; attempt++) {
+      for (uint64_t todo = looking; todo;) {  // one claim per distinct class
+        const uint32_t lead_tree = wave::readlane(tree, wave::first_lane(todo));
+        const uint64_t group = wave::ballot(tree == lead_tree) & todo;
+        if ((group >> wave::lane_id()) & 1ull) claim_group(group, lead_tree, count, tid, hint, result, done, oom);
+        todo &= ~group;
       }
-      if (done) break;
+      looking &= ~wave::ballot(done);
     }
     return result;
   }
@@ -326,9 +360,10 @@ struct Gallatin {
     Block* b = &table.blocks[block];
     uint32_t old_merged = atom::add(&b->malloc_counter, kSlicesPerBlock);
     uint32_t old_count = old_merged & kBlockCountMask;
-    if ((old_merged >> kBlockTreeShift) != tree) count_error();
+    if ((old_merged >> kBlockTreeShift) != tree) count_error(kErrFillClass);
     atom::add(&b->free_counter, 4095u - old_count);
     if (old_count != 0) {
+      count_error(kErrFillCount);
       free_slices((uint64_t)block, 1u);
       return utils::kFail;
     }
@@ -344,7 +379,7 @@ struct Gallatin {
     atom::fence_acquire();
     uint64_t first = segment_tree.gather_multiple(n);
     if (first != utils::kFail) {
-      if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error();
+      if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error(kErrSetup);
     }
     atom::fence_release();
     release_tree_lock(kNumTrees);
@@ -391,8 +426,9 @@ struct Gallatin {
       }
     }
     uint64_t offset = utils::kFail;
-    for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail; attempt++)
-      offset = malloc_slice_allocation(tree, count, tid, hint);
+    bool oom = false;
+    for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail && !oom; attempt++)
+      offset = malloc_slice_allocation(tree, count, tid, hint, oom);
     if (offset == utils::kFail) return nullptr;
     return offset_to_allocation(offset, tree);
   }
@@ -407,13 +443,12 @@ struct Gallatin {
   // slices of one block freed together by the lanes of a wave
   __device__ void free_offset(uint64_t offset) {
     const uint64_t block = offset / kSlicesPerBlock;
-    for (;;) {
-      const uint64_t lead = wave::readfirst64(block);
-      if (block == lead) {
-        const uint64_t group = wave::active_mask();
-        if ((int)wave::lane_id() == wave::first_lane(group)) free_slices(lead, wave::popc(group));
-        break;
-      }
+    for (uint64_t todo = wave::active_mask(); todo;) {  // uniform loop, see malloc_slice_allocation
+      const int first = wave::first_lane(todo);
+      const uint64_t lead = wave::readlane64(block, first);
+      const uint64_t group = wave::ballot(block == lead) & todo;
+      if ((int)wave::lane_id() == first) free_slices(lead, wave::popc(group));
+      todo &= ~group;
     }
   }
 
@@ -433,7 +468,7 @@ struct Gallatin {
     if (tree > kNumTrees) {
       // a run of segments: release the id first so that no claimer can find
       // the segment in the tree while the old id is still set
-      if (!table.reset_tree_id(segment, tree)) count_error();
+      if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
       atom::fence_release();
       segment_tree.return_multiple(segment, tree - kNumTrees - 1);
       return;

@@ -38,11 +38,34 @@ struct segment_counters {
 // device error counters (reference traps), kept in the control word array
 enum control_word : uint32_t {
   kCtlLocks = 0,          // bit c: class c is attaching a segment; bit T: segment claim
-  kCtlErrors = 1,         // internal inconsistencies observed
+  kCtlErrors = 1,         // internal inconsistencies observed (sum of the sites below)
   kCtlBadFrees = 2,       // frees of pointers the allocator does not own
   kCtlBootStatus = 3,     // 0 booting, 1 ok, 2 failed
-  kCtlWords = 16
+  kCtlSiteBase = 4,       // per-site counts of internal inconsistencies:
+  kErrQueueEmpty = 0,     //   a grant found its recycle-queue slot empty
+  kErrResetCount = 1,     //   a completed block's free count was not 4096 at reset
+  kErrChunkReset = 2,     //   a released segment's class id changed under it
+  kErrSetup = 3,          //   a free segment could not be formatted
+  kErrQueueSpin = 4,      //   a queue write waited too long for its slot
+  kErrFinalSpin = 5,      //   a queue write waited too long for earlier writes
+  kErrFillClass = 6,      //   a whole-block grant carried another class
+  kErrRunReset = 7,       //   a segment run's id changed under its free
+  kErrFillCount = 8,      //   a whole-block grant found slices already claimed
+  kErrInitCounter = 9,    //   a granted block's counter held the impossible value
+  kErrGrantClass = 10,    //   a block was granted from a segment of another class
+  kErrClaimClass = 11,    //   a slice was claimed from a segment of another class
+  kErrFreeClass = 12,     //   a free hit a block whose counter carries another class
+  kErrOverReturn = 13,    //   a segment got back more blocks than it granted
+  kErrSites = 14,
+  kCtlWords = 32
 };
+
+#if defined(__HIPCC__)
+__device__ inline void count_error_at(uint32_t* control, uint32_t site) {
+  atom::add(&control[kCtlErrors], 1u);
+  atom::add(&control[kCtlSiteBase + site], 1u);
+}
+#endif
 
 template <uint64_t SEG, uint64_t MIN>
 struct alloc_table {
@@ -94,11 +117,16 @@ struct alloc_table {
   __device__ int64_t get_block(uint64_t segment, uint32_t tree, bool& last, uint32_t* control) {
     last = false;
     segment_counters* s = &segs[segment];
-    int32_t active_count = atom::sub(&s->active, 1);
-    if (active_count < 0) {
-      atom::add(&s->active, 1);
-      return -1;
+    // Take one grant without ever driving the count below -1: an undo
+    // increment after a failed decrement could land after setup_segment has
+    // reset the count for the segment's next use and over-admit it.
+    int32_t active_count = atom::load(&s->active);
+    while (active_count >= 0) {
+      const int32_t seen = atom::cas(&s->active, active_count, active_count - 1);
+      if (seen == active_count) break;
+      active_count = seen;
     }
+    if (active_count < 0) return -1;
     if (read_tree_id(segment) != tree) {
       atom::add(&s->active, 1);
       return -1;
@@ -111,12 +139,16 @@ struct alloc_table {
     } else {
       uint32_t v = atom::exch(&queues[segment * blocks_per_segment + (uint64_t)queue_pos % nb], 0u);
       if (v == 0u) {  // the reference traps here; count it instead
-        atom::add(&control[kCtlErrors], 1u);
+        count_error_at(control, kErrQueueEmpty);
         return -1;
       }
       block = (int64_t)v - 1;
     }
-    atom::exch(&blocks[block].malloc_counter, tree << kBlockTreeShift);  // Block::init_malloc
+    // Block::init_malloc. The old value is read so that the reset is
+    // performed before the block can be published anywhere.
+    const uint32_t old_counter = atom::exch(&blocks[block].malloc_counter, tree << kBlockTreeShift);
+    if (old_counter == ~0u) count_error_at(control, kErrInitCounter);
+    if (read_tree_id(segment) != tree) count_error_at(control, kErrGrantClass);
     if (active_count == 0) last = true;
     return block;
   }
@@ -128,13 +160,26 @@ struct alloc_table {
     segment_counters* s = &segs[segment];
     uint32_t pos = atom::add(&s->queue_free, 1u);
     uint32_t live = (uint32_t)(pos % nb);
-    atom::exch(&queues[segment * blocks_per_segment + live], (uint32_t)(block + 1));
+    // The slot still holds the entry of `pos - nb` when the grant that owns
+    // that entry has taken its queue position but not yet read the slot: an
+    // exchange would overwrite (lose) that block. Wait for the read instead.
+    // The reader is never in this wave (a wave runs one allocator path at a
+    // time and get_block reads its slot without waiting on anything).
+    uint32_t* q = &queues[segment * blocks_per_segment + live];
+    for (uint32_t spin = 0; atom::cas(q, 0u, (uint32_t)(block + 1)) != 0u; spin++) {
+      if (spin > (1u << 24)) {
+        count_error_at(control, kErrQueueSpin);
+        atom::exch(q, (uint32_t)(block + 1));
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
     // Earlier reservations are held by other waves (a wave completes its own
     // before it can take another), so this wait always ends; the bound only
     // turns a broken invariant into a counted error instead of a hang.
     for (uint32_t spin = 0; atom::cas(&s->queue_final, pos, pos + 1) != pos; spin++) {
       if (spin > (1u << 24)) {
-        atom::add(&control[kCtlErrors], 1u);
+        count_error_at(control, kErrFinalSpin);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -144,9 +189,10 @@ struct alloc_table {
 
   // reference memory_table.cuh:1226-1241; true when this return makes every
   // block of the segment free and this lane won the right to release it
-  __device__ bool finish_freeing_block(uint64_t segment, uint64_t nb) {
+  __device__ bool finish_freeing_block(uint64_t segment, uint64_t nb, uint32_t* control) {
     atom::fence_release();  // the queue entry is visible before the grant is
     int32_t return_id = atom::add(&segs[segment].active, 1);
+    if ((int64_t)return_id > (int64_t)nb - 2) count_error_at(control, kErrOverReturn);
     if ((int64_t)return_id == (int64_t)nb - 2) {
       if (atom::cas(&segs[segment].active, (int32_t)nb - 1, (int32_t)-1) == (int32_t)nb - 1) return true;
     }

@@ -117,9 +117,18 @@ struct veb_tree {
     return utils::kFail;
   }
 
+  // true when the tree holds no item as seen from the top summary. Exact for
+  // trees changed only by insert_force_update/remove (their summaries track
+  // the item layer); the segment tree's runs can leave stale summaries.
+  __device__ bool looks_empty() const { return load_word(0, 0) == 0 && (num_layers > 1 || words[0] == 1); }
+
+  // nothing malloc_first could claim: no item 0 and no successor of 0
+  __device__ bool nothing_to_claim() const { return !query(0) && successor_thorough(0) == utils::kFail; }
+
   // hashed start positions (reference veb.cuh:946-977); tid is the caller's
   // global thread id
   __device__ uint64_t find_random_valid_index(uint64_t tid) const {
+    if (looks_empty()) return utils::kFail;
     uint64_t hash1 = utils::murmur64a(tid, seed);
     uint64_t hash2 = utils::murmur64a(tid, hash1);
     for (int attempts = 0; attempts < kVebHashedAttempts; attempts++) {

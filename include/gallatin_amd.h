@@ -116,6 +116,16 @@ int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* result
 /* Device error counters: internal inconsistencies (where the reference would
  * trap) and rejected frees of pointers the allocator does not own. */
 int gal_errors(gal_allocator* h, uint64_t* internal_errors, uint64_t* bad_frees);
+/* Per-site counts behind internal_errors, in this order: recycle-queue slot
+ * empty at a grant; block free count not 4096 at its reset; class id of a
+ * released segment changed; a free segment could not be formatted; queue
+ * write waited too long for its slot; queue write waited too long for earlier
+ * writes; whole-block grant of another class; segment-run id changed under its
+ * free; whole-block grant found slices already claimed; impossible counter at
+ * a grant; grant from a segment of another class; slice claim from a segment
+ * of another class; free of a block carrying another class; more block
+ * returns than grants. Writes min(cap, 14) counts. */
+int gal_error_sites(gal_allocator* h, uint32_t* counts, uint32_t cap);
 
 /* Global allocator (Gallatin<16 MiB, 16, 4096> singleton).
  * Replaces init_global_allocator / free_global_allocator / print_global_stats,

@@ -477,6 +477,9 @@ struct Table {
   uint32_t reserve_segment_slot(uint64_t block, uint64_t seg, uint64_t nb, uint64_t& faults) {
     uint32_t pos = q_free[seg]++;
     uint32_t live = (uint32_t)(pos % nb);
+    // with one lane the slot's previous entry has always been taken: a segment
+    // with all nb blocks queued has no block left to return
+    if (queues[seg * bps + live] != 0u) faults++;
     queues[seg * bps + live] = (uint32_t)(block + 1);
     if (q_final[seg] != pos) faults++;  // the CAS spin would never end
     q_final[seg] = pos + 1;

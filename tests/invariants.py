@@ -48,7 +48,17 @@ def check_live(offsets, sizes, pool_bytes, seg, mn, mx, route):
     order = np.argsort(offsets, kind="stable")
     o, g = offsets[order], granted[order]
     overlap = np.nonzero(o[:-1] + g[:-1] > o[1:])[0]
-    assert len(overlap) == 0, f"{len(overlap)} overlapping allocations, first at offset {int(o[overlap[0]])}"
+    if len(overlap):
+        sz = sizes[order]
+        pairs = [
+            (int(o[k]), int(sz[k]), int(o[k + 1]), int(sz[k + 1]), int(order[k]), int(order[k + 1]))
+            for k in overlap[:8]
+        ]
+        segs = sorted({int(o[k]) // seg for k in overlap})
+        raise AssertionError(
+            f"{len(overlap)} overlapping allocations in segments {segs[:16]}; first pairs "
+            f"(off, size, off, size, idx, idx): {pairs}"
+        )
 
 
 def bits_of(words):
@@ -89,7 +99,15 @@ def check_quiescent(image, seg_bytes, min_size):
             assert active[x] == nb - 1 - qa[x] + qf[x], f"segment {x}: active count off"
             assert qfin[x] == qf[x], f"segment {x}: unfinished queue write"
             outstanding = nb - 1 - active[x]
-            assert outstanding == pinned_per_seg[x], f"segment {x}: {outstanding} blocks out, {pinned_per_seg[x]} pinned"
+            if outstanding != pinned_per_seg[x]:
+                lo = x * bps
+                blk = [(i, int(s["malloc_counter"][lo + i]), int(s["free_counter"][lo + i])) for i in range(nb)]
+                raise AssertionError(
+                    f"segment {x} (class {c}, nb {nb}): {outstanding} blocks out, {pinned_per_seg[x]} pinned; "
+                    f"active {active[x]} q_alloc {qa[x]} q_free {qf[x]} q_final {qfin[x]}; "
+                    f"queue {s['queues'][lo:lo + nb].tolist()}; pinned here "
+                    f"{sorted(int(b - lo) for b in pinned if b // bps == x)}; blocks (idx, malloc, free) {blk}"
+                )
         elif c == 0xFFFF:
             assert active[x] == -1, f"free segment {x} has active {active[x]}"
             assert pinned_per_seg[x] == 0
