@@ -93,7 +93,7 @@ def main():
         for r in range(rounds):
             res = g.mixed_round(r)
             log(f"[bench] {tag} round {r}: mallocs {res['mallocs']} frees {res['frees']} misses {res['misses']} "
-                f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f}")
+                f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f} stats {g.stats()}")
             total = res if total is None else {k: total[k] + res[k] for k in total}
         res = g.mixed_drain()
         total = {k: total[k] + res[k] for k in total}

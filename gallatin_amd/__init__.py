@@ -126,6 +126,13 @@ class Gallatin:
         _capi.check("gal_error_sites", _lib.gal_error_sites(self._h, buf, 14))
         return {name: buf[i] for i, name in enumerate(self.ERROR_SITES) if buf[i]}
 
+    STAT_NAMES = ("replace", "refill", "request", "request_retry", "attach", "release", "recycle")
+
+    def stats(self):
+        buf = (ctypes.c_uint32 * 7)()
+        _capi.check("gal_stats", _lib.gal_stats(self._h, buf, 7))
+        return {name: buf[i] for i, name in enumerate(self.STAT_NAMES)}
+
     def snapshot(self):
         need = ctypes.c_uint64()
         _lib.gal_snapshot(self._h, None, 0, ctypes.byref(need))

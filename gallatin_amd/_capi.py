@@ -86,6 +86,7 @@ _SIGNATURES = {
     "gal_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64)]),
     "gal_errors": (ctypes.c_int, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
     "gal_error_sites": (ctypes.c_int, [vp, ctypes.POINTER(u32), u32]),
+    "gal_stats": (ctypes.c_int, [vp, ctypes.POINTER(u32), u32]),
     "gal_init_global_allocator": (ctypes.c_int, [u64, u64, ctypes.c_int]),
     "gal_free_global_allocator": (ctypes.c_int, []),
     "gal_print_global_stats": (ctypes.c_int, []),

@@ -503,6 +503,15 @@ int gal_error_sites(gal_allocator* h, uint32_t* counts, uint32_t cap) {
   return GAL_OK;
 }
 
+int gal_stats(gal_allocator* h, uint32_t* counts, uint32_t cap) {
+  if (!h || !counts) return GAL_ERR_INVALID;
+  uint32_t ctl[ga::kCtlWords];
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  if (hipMemcpy(ctl, h->control, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  for (uint32_t i = 0; i < cap && i < ga::kStatSites; i++) counts[i] = ctl[ga::kCtlStatBase + i];
+  return GAL_OK;
+}
+
 int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* results) {
   if (!h || (n && (!ops || !results))) return GAL_ERR_INVALID;
   if (n == 0) return GAL_OK;

@@ -166,9 +166,36 @@ __device__ inline uint64_t readlane64(uint64_t v, int lane) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// value of `v` in lane `src` (per-lane source; the source lane must be active)
+__device__ inline uint32_t shfl(uint32_t v, int src) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ inline uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = shfl((uint32_t)v, src);
+  const uint32_t hi = shfl((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Partition the lanes of `lanes` by equal `key`; returns this lane's group
+// (a lane mask). A uniform loop with one iteration per distinct key, using
+// only register and scalar work.
+__device__ inline uint64_t match_groups(uint64_t lanes, uint64_t key) {
+  uint64_t mine = 0;
+  const uint32_t me = lane_id();
+  for (uint64_t todo = lanes; todo;) {
+    const uint64_t k = readlane64(key, first_lane(todo));
+    const uint64_t g = ballot(key == k) & todo;
+    mine = ((g >> me) & 1ull) ? g : mine;
+    todo &= ~g;
+  }
+  return mine;
+}
+
 // Exclusive prefix sum and total of a per-lane count over the lanes of
 // `group`, in lane order, built from ballots of the count's bits so that no
 // lane outside the group is ever read. Counts are < 2^bits.
+// `group` may differ per lane (each lane passes its own group's mask), so
+// several groups are scanned at once.
 __device__ inline void group_scan(uint64_t group, uint32_t count, int bits, uint32_t& prefix, uint32_t& total) {
   uint64_t below = group & lanemask_lt();
   uint32_t p = 0, t = 0;

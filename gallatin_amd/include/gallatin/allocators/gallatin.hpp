@@ -105,19 +105,6 @@ struct Gallatin {
 
   __device__ uint32_t* slot_ptr(uint32_t tree, uint32_t index) const { return &slots[slot_offset[tree] + index]; }
 
-  // start at hint % n and step past empty slots, at most 30 steps
-  // (reference shared_block_storage.cuh:119-134)
-  __device__ uint32_t valid_block_index(uint32_t tree, uint32_t hint) const {
-    const int n = (int)pinned_slots_of(tree);
-    int my = (int)(hint % (uint32_t)n);
-    const int original = my;
-    for (int counter = 0; atom::load(slot_ptr(tree, (uint32_t)my)) == 0u && my != original - 1;) {
-      my = (my + 1) % n;
-      if (++counter >= kPinnedWalk) break;
-    }
-    return (uint32_t)my;
-  }
-
   // ------------------------------------------------- segments and blocks
   // attach the lowest free segment to class `tree` (reference :1465-1510)
   __device__ __noinline__ int64_t gather_new_segment(uint32_t tree) {
@@ -127,6 +114,7 @@ struct Gallatin {
       count_error(kErrSetup);
       return -1;
     }
+    count_stat(control, kStatAttach);
     atom::fence();
     sub_trees[tree].insert_force_update(segment);
     atom::fence();
@@ -141,7 +129,9 @@ struct Gallatin {
     // toward giving up; passes that lose a race (the class lock is held, the
     // segment found is exhausted) retry, up to a large safety bound.
     int attempts = 0;
+    count_stat(control, kStatRequest);
     for (int pass = 0; attempts < kRequestBlockAttempts && pass < kRequestBlockPasses; pass++) {
+      if (pass > 0) count_stat(control, kStatRequestRetry);
       uint64_t segment = sub_trees[tree].find_random_valid_index(tid);
       if (segment == utils::kFail) {
         // out of memory is confirmed without the class lock: gather_new_segment
@@ -197,6 +187,7 @@ struct Gallatin {
     const bool release = table.finish_freeing_block(segment, nb, control);
     atom::fence();
     if (release) {
+      count_stat(control, kStatRelease);
       sub_trees[tree].remove(segment);
       atom::fence();
       if (!table.reset_tree_id(segment, tree)) count_error(kErrChunkReset);
@@ -210,13 +201,11 @@ struct Gallatin {
     return_block(block, segment, table.read_tree_id(segment));
   }
 
-  // account n returned slices of `block`; the return that completes 4096
-  // recycles the block (reference block.cuh:87-98, gallatin.cuh:1790-1799)
+  // account n returned slices of `block` (one lane); the return that
+  // completes 4096 recycles the block (reference block.cuh:87-98,
+  // gallatin.cuh:1790-1799)
   __device__ void free_slices(uint64_t block, uint32_t n) {
-    Block* b = &table.blocks[block];
-    if ((atom::load(&b->malloc_counter) >> kBlockTreeShift) != table.read_tree_id(block / kBlocksPerSegment))
-      count_error(kErrFreeClass);
-    uint32_t old = atom::add_release(&b->free_counter, n);
+    const uint32_t old = atom::add(&table.blocks[block].free_counter, n);
     if (old + n == kSlicesPerBlock) recycle_block(block);
   }
 
@@ -227,6 +216,7 @@ struct Gallatin {
     // block is queued, or a late reset would wipe its next owner's count
     const uint32_t prev = atom::exch(&table.blocks[block].free_counter, 0u);
     if (prev != kSlicesPerBlock) count_error(kErrResetCount);
+    count_stat(control, kStatRecycle);
     free_block(block);
   }
 
@@ -235,6 +225,7 @@ struct Gallatin {
   __device__ __noinline__ bool replace_block(uint32_t tree, uint32_t index, uint64_t block, uint64_t tid) {
     uint32_t* s = slot_ptr(tree, index);
     if (atom::cas(s, (uint32_t)(block + 1), 0u) != (uint32_t)(block + 1)) return true;
+    count_stat(control, kStatReplace);
     atom::fence();
     int64_t fresh = request_new_block_from_tree(tree, tid);
     if (fresh < 0) return false;
@@ -248,6 +239,7 @@ struct Gallatin {
 
   // an empty pinned slot (left so by an out-of-memory replace) is refilled
   __device__ __noinline__ bool refill_slot(uint32_t tree, uint32_t index, uint64_t tid) {
+    count_stat(control, kStatRefill);
     int64_t fresh = request_new_block_from_tree(tree, tid);
     if (fresh < 0) return false;
     atom::fence_release();
@@ -255,86 +247,120 @@ struct Gallatin {
     return true;
   }
 
-  // ------------------------------------------------------- slice claims
-  // One claim for the lanes of the current exec mask (all of class `tree`).
-  // Sets done/result for the lanes that are finished.
-  __device__ void claim_group(uint64_t group, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
-                              uint64_t& result, bool& done, bool& oom) {
-    const int leader = wave::first_lane(group);
-    const bool is_leader = (int)wave::lane_id() == leader;
-    const uint32_t lead_hint = wave::readlane(hint, leader);
-    const uint64_t lead_tid = wave::readlane64(tid, leader);
-
-    const uint32_t index = valid_block_index(tree, lead_hint);
-    const uint32_t entry = wave::readlane(atom::load(slot_ptr(tree, index)), leader);
-    if (entry == 0u) {
-      bool ok = false;
-      if (is_leader) ok = refill_slot(tree, index, lead_tid);
-      if ((wave::ballot(ok) & group) == 0) {  // class out of memory
-        done = true;
-        oom = true;
-        result = utils::kFail;
-      }
-      return;
+  // the pinned slot a group uses: start at hint % n, step past empty slots
+  // (at most 30 steps, reference shared_block_storage.cuh:119-145); returns
+  // the slot's index and the entry read there
+  __device__ void pinned_slot(uint32_t tree, uint32_t hint, uint32_t& index, uint32_t& entry) const {
+    const int n = (int)pinned_slots_of(tree);
+    int my = (int)(hint % (uint32_t)n);
+    const int original = my;
+    entry = atom::load(slot_ptr(tree, (uint32_t)my));
+    for (int counter = 0; entry == 0u && my != original - 1;) {
+      my = (my + 1) % n;
+      entry = atom::load(slot_ptr(tree, (uint32_t)my));
+      if (++counter >= kPinnedWalk) break;
     }
-    const uint64_t block = entry - 1u;
-    Block* b = &table.blocks[block];
+    index = (uint32_t)my;
+  }
 
-    // exclusive scan of the slice counts in lane order
+  // ------------------------------------------------------- slice claims
+  // One claim pass for the lanes of `looking` (the caller's exec mask),
+  // reference gallatin.cuh:865-1078. Lanes asking for the same class form a
+  // group; ALL groups of the wave claim at once: one slot-load instruction,
+  // one atomic instruction in which each group's last lane adds the group's
+  // slice total to its own block's malloc counter, and a lane shuffle that
+  // hands each lane its group's old count. Rare per-group work (refilling an
+  // empty slot, replacing a spent block, recycling a written-off block,
+  // returning slices drawn from a stale block) runs one group at a time.
+  __device__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
+                             uint64_t& result, bool& done, bool& oom) {
+    const uint32_t me = wave::lane_id();
+    const uint64_t group = wave::match_groups(looking, tree);
+    const int leader = wave::first_lane(group);
+    const int last = wave::last_lane(group);
+    const bool is_leader = (int)me == leader;
+
+    uint32_t index, entry;
+    pinned_slot(tree, wave::shfl(hint, leader), index, entry);
+    index = wave::shfl(index, leader);
+    entry = wave::shfl(entry, leader);
+    const bool has_block = entry != 0u;
+    const uint64_t block = has_block ? entry - 1u : 0;
+
+    // exclusive scan of the slice counts inside each group, in lane order
     uint32_t prefix, total;
-    if ((wave::ballot(count != 1u) & group) == 0) {
+    if ((wave::ballot(count != 1u) & looking) == 0) {
       prefix = wave::popc(group & wave::lanemask_lt());
       total = wave::popc(group);
     } else {
       wave::group_scan(group, count, 13, prefix, total);
     }
-    // the group's last lane holds the inclusive total and does the one add
+    // each group's last lane holds the inclusive total and does the one add
     // (reference block.cuh:245-264)
-    const int last = wave::last_lane(group);
     uint32_t merged = 0;
-    if ((int)wave::lane_id() == last) merged = atom::add(&b->malloc_counter, total);
-    merged = wave::readlane(merged, last);
+    if ((int)me == last && has_block) merged = atom::add(&table.blocks[block].malloc_counter, total);
+    merged = wave::shfl(merged, last);
 
     const uint32_t c0 = merged & kBlockCountMask;
-    uint64_t allocation = (uint64_t)c0 + prefix;
-    const bool should_replace = allocation <= 4095 && allocation + count > 4095;
-    const bool valid = allocation + count <= kSlicesPerBlock;
+    const uint64_t allocation = (uint64_t)c0 + prefix;
+    const bool should_replace = has_block && allocation <= 4095 && allocation + count > 4095;
+    const bool valid = has_block && allocation + count <= kSlicesPerBlock;
+    const uint64_t valid_lanes = wave::ballot(valid) & group;
     // Slices of the claimed range that lie inside the block but belong to no
     // successful lane (the extra slices of multi-slice requests and a
     // straddling request's tail) are written off as already freed
     // (reference block.cuh:267-281): in-range slices minus successful lanes.
-    const uint32_t in_range = c0 >= kSlicesPerBlock ? 0u : ((c0 + total < kSlicesPerBlock ? c0 + total : kSlicesPerBlock) - c0);
-    const uint64_t valid_lanes = wave::ballot(valid) & group;
-    const uint32_t written_off = in_range - wave::popc(valid_lanes);
+    const uint32_t in_range =
+        c0 >= kSlicesPerBlock ? 0u : ((c0 + total < kSlicesPerBlock ? c0 + total : kSlicesPerBlock) - c0);
+    const uint32_t written_off = has_block ? in_range - wave::popc(valid_lanes) : 0u;
     bool completed = false;
-    if (written_off > 0 && is_leader) {
-      uint32_t old = atom::add(&b->free_counter, written_off);
+    if (is_leader && written_off > 0) {
+      const uint32_t old = atom::add(&table.blocks[block].free_counter, written_off);
       completed = (old + written_off == kSlicesPerBlock);
     }
-    completed = (wave::ballot(completed) & group) != 0;
-    if ((wave::ballot(should_replace) & group) != 0 && is_leader) replace_block(tree, index, block, lead_tid);
-    if (completed && is_leader) recycle_block(block);  // the write-off finished the block
-    if ((merged >> kBlockTreeShift) != tree) {
-      // a stale pinned read: the block now serves another class; give back
-      // what the valid lanes drew (reference gallatin.cuh:1037-1056)
-      const uint32_t drawn = wave::popc(valid_lanes);
-      if (drawn > 0 && is_leader) free_slices(block, drawn);
-      return;
+    // a stale pinned read: the block now serves another class; what the valid
+    // lanes drew is given back (reference gallatin.cuh:1037-1056)
+    const bool stale = has_block && (merged >> kBlockTreeShift) != tree;
+    const uint32_t give_back = stale ? wave::popc(valid_lanes) : 0u;
+    // The group whose range crosses slice 4095 replaces the block (reference
+    // :992, :1015-1026). A group that finds the block already spent, or
+    // serving another class, replaces it too: a claim made through a stale
+    // slot read can exhaust a block that was recycled into another slot, and
+    // its replace attempt then targets the old slot, leaving the new one
+    // pinned to a spent block. With one lane this never happens (the crossing
+    // lane always swaps its own slot), so single-lane results are unchanged.
+    const bool replace = ((wave::ballot(should_replace) & group) != 0) || (has_block && c0 > 4095) || stale;
+
+    bool refilled = true;
+    const uint64_t rare = wave::ballot(is_leader && (!has_block || replace || completed || give_back > 0)) & looking;
+    for (uint64_t todo = rare; todo; todo &= todo - 1) {
+      if ((int)me == wave::first_lane(todo)) {
+        const uint64_t lead_tid = tid;
+        if (!has_block) {
+          refilled = refill_slot(tree, index, lead_tid);
+        } else {
+          if (replace) replace_block(tree, index, block, lead_tid);
+          if (completed) recycle_block(block);  // the write-off finished the block
+          if (give_back > 0) free_slices(block, give_back);
+        }
+      }
     }
-    if (valid) {
+    if (!has_block && wave::shfl((uint32_t)refilled, leader) == 0u) {  // class out of memory
+      done = true;
+      oom = true;
+      result = utils::kFail;
+    }
+    if (valid && !stale) {
       result = allocation + block * kSlicesPerBlock;
       done = true;
     }
-    if (is_leader && (valid_lanes != 0) && table.read_tree_id(block / kBlocksPerSegment) != tree)
-      count_error(kErrClaimClass);
   }
 
-  // reference gallatin.cuh:865-1078 for every lane of the wave that calls it.
-  // The loops are uniform (driven by lane masks, never by a lane leaving the
-  // loop): the AMDGPU structurizer runs a loop's exit block once, after the
-  // loop, for all lanes that left, so per-group work must not sit in one.
-  // `oom` is set when the failure is a confirmed out-of-memory, which a
-  // further call would only repeat.
+  // Slice claims for every lane of the wave that calls it. The loop is
+  // uniform (driven by a lane mask, never by a lane leaving it): the AMDGPU
+  // structurizer runs a loop's exit block once, after the loop, for all lanes
+  // that left, so per-group work must not sit in one. `oom` is set when the
+  // failure is a confirmed out-of-memory, which a further call would repeat.
   __device__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
                                               bool& oom) {
     uint64_t result = utils::kFail;
@@ -342,12 +368,7 @@ struct Gallatin {
     oom = false;
     uint64_t looking = wave::active_mask();
     for (int attempt = 0; attempt < kSliceAttempts && looking; attempt++) {
-      for (uint64_t todo = looking; todo;) {  // one claim per distinct class
-        const uint32_t lead_tree = wave::readlane(tree, wave::first_lane(todo));
-        const uint64_t group = wave::ballot(tree == lead_tree) & todo;
-        if ((group >> wave::lane_id()) & 1ull) claim_group(group, lead_tree, count, tid, hint, result, done, oom);
-        todo &= ~group;
-      }
+      if ((looking >> wave::lane_id()) & 1ull) claim_pass(looking, tree, count, tid, hint, result, done, oom);
       looking &= ~wave::ballot(done);
     }
     return result;
@@ -440,16 +461,24 @@ struct Gallatin {
   }
 
   // ----------------------------------------------------------------- free
-  // slices of one block freed together by the lanes of a wave
+  // Frees of the lanes of a wave, grouped by block: one release fence for the
+  // wave (its stores are performed before any of its slices can be handed
+  // out again), then one atomic instruction in which each group's first lane
+  // adds the group's size to its block's free counter. A block whose count
+  // completes is recycled, one at a time.
   __device__ void free_offset(uint64_t offset) {
     const uint64_t block = offset / kSlicesPerBlock;
-    for (uint64_t todo = wave::active_mask(); todo;) {  // uniform loop, see malloc_slice_allocation
-      const int first = wave::first_lane(todo);
-      const uint64_t lead = wave::readlane64(block, first);
-      const uint64_t group = wave::ballot(block == lead) & todo;
-      if ((int)wave::lane_id() == first) free_slices(lead, wave::popc(group));
-      todo &= ~group;
+    const uint64_t group = wave::match_groups(wave::active_mask(), block);
+    const bool is_leader = (int)wave::lane_id() == wave::first_lane(group);
+    atom::fence_release();
+    bool complete = false;
+    if (is_leader) {
+      const uint32_t n = wave::popc(group);
+      const uint32_t old = atom::add(&table.blocks[block].free_counter, n);
+      complete = (old + n == kSlicesPerBlock);
     }
+    for (uint64_t todo = wave::ballot(complete); todo; todo &= todo - 1)
+      if ((int)wave::lane_id() == wave::first_lane(todo)) recycle_block(block);
   }
 
   // reference gallatin.cuh:1378-1461

@@ -57,6 +57,15 @@ enum control_word : uint32_t {
   kErrFreeClass = 12,     //   a free hit a block whose counter carries another class
   kErrOverReturn = 13,    //   a segment got back more blocks than it granted
   kErrSites = 14,
+  kCtlStatBase = 20,      // activity counters of the rare paths:
+  kStatReplace = 0,       //   spent pinned blocks swapped out
+  kStatRefill = 1,        //   empty pinned slots refilled
+  kStatRequest = 2,       //   block requests (request_new_block_from_tree)
+  kStatRequestRetry = 3,  //   block-request passes that lost a race
+  kStatAttach = 4,        //   segments attached to a class
+  kStatRelease = 5,       //   segments released to the segment tree
+  kStatRecycle = 6,       //   blocks recycled after all slices came back
+  kStatSites = 7,
   kCtlWords = 32
 };
 
@@ -65,6 +74,7 @@ __device__ inline void count_error_at(uint32_t* control, uint32_t site) {
   atom::add(&control[kCtlErrors], 1u);
   atom::add(&control[kCtlSiteBase + site], 1u);
 }
+__device__ inline void count_stat(uint32_t* control, uint32_t stat) { atom::add(&control[kCtlStatBase + stat], 1u); }
 #endif
 
 template <uint64_t SEG, uint64_t MIN>

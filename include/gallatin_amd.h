@@ -126,6 +126,12 @@ int gal_errors(gal_allocator* h, uint64_t* internal_errors, uint64_t* bad_frees)
  * of another class; free of a block carrying another class; more block
  * returns than grants. Writes min(cap, 14) counts. */
 int gal_error_sites(gal_allocator* h, uint32_t* counts, uint32_t cap);
+/* Cumulative activity of the allocator's rare paths, in this order: spent
+ * pinned blocks swapped out; empty pinned slots refilled; block requests;
+ * block-request passes that lost a race and retried; segments attached to a
+ * class; segments released to the segment tree; blocks recycled after all
+ * their slices came back. Writes min(cap, 7) counts. */
+int gal_stats(gal_allocator* h, uint32_t* counts, uint32_t cap);
 
 /* Global allocator (Gallatin<16 MiB, 16, 4096> singleton).
  * Replaces init_global_allocator / free_global_allocator / print_global_stats,
