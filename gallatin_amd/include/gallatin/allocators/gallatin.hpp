@@ -43,6 +43,9 @@ constexpr int kBlockAttempts = 4;           // GALLATIN_MALLOC_BLOCK_ATTEMPTS (5
 constexpr int kSegmentAttempts = 4;         // GALLATIN_MALLOC_SEGMENT_ATTEMPTS (500) on OOM
 constexpr int kPinnedWalk = 30;             // SHARED_BLOCK_COUNTER_CUTOFF
 constexpr uint32_t kPinnedSlotsClass0 = 128;
+// a pinned slot holds 0 (empty: a replacement ran out of memory), a block id
+// + 1, or kSlotBusy while one lane replaces or refills it
+constexpr uint32_t kSlotBusy = 0xFFFFFFFFu;
 constexpr uint32_t kPinnedSlotsMin = 4;     // MIN_PINNED_CUTOFF
 
 __host__ __device__ constexpr int ilog2(uint64_t x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
@@ -220,31 +223,38 @@ struct Gallatin {
     free_block(block);
   }
 
-  // swap a spent pinned block for a fresh one (reference :1083-1138); a
-  // refill that loses the race hands its block back
+  // Swap a spent pinned block for a fresh one (reference :1083-1138). The
+  // slot is marked busy while the new block is found, so that other groups
+  // wait for it instead of racing to refill it; it is left empty when memory
+  // runs out.
   __device__ __noinline__ bool replace_block(uint32_t tree, uint32_t index, uint64_t block, uint64_t tid) {
     uint32_t* s = slot_ptr(tree, index);
-    if (atom::cas(s, (uint32_t)(block + 1), 0u) != (uint32_t)(block + 1)) return true;
+    if (atom::cas(s, (uint32_t)(block + 1), kSlotBusy) != (uint32_t)(block + 1)) return true;
     count_stat(control, kStatReplace);
     atom::fence();
-    int64_t fresh = request_new_block_from_tree(tree, tid);
-    if (fresh < 0) return false;
+    const int64_t fresh = request_new_block_from_tree(tree, tid);
     atom::fence_release();  // the block's counter reset precedes its publication
-    if (atom::cas(s, 0u, (uint32_t)(fresh + 1)) != 0u) {
-      free_block((uint64_t)fresh);
-      return false;
-    }
-    return true;
+    atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
+    return fresh >= 0;
   }
 
-  // an empty pinned slot (left so by an out-of-memory replace) is refilled
+  // refill an empty slot (left so by an out-of-memory replace); false when
+  // memory is still short
   __device__ __noinline__ bool refill_slot(uint32_t tree, uint32_t index, uint64_t tid) {
+    uint32_t* s = slot_ptr(tree, index);
+    if (atom::cas(s, 0u, kSlotBusy) != 0u) return true;  // another group is on it
     count_stat(control, kStatRefill);
-    int64_t fresh = request_new_block_from_tree(tree, tid);
-    if (fresh < 0) return false;
+    const int64_t fresh = request_new_block_from_tree(tree, tid);
     atom::fence_release();
-    if (atom::cas(slot_ptr(tree, index), 0u, (uint32_t)(fresh + 1)) != 0u) free_block((uint64_t)fresh);
-    return true;
+    atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
+    return fresh >= 0;
+  }
+
+  // nothing a block request could find: no listed segment for the class and
+  // no free segment (with one lane, exactly when request_new_block_from_tree
+  // fails)
+  __device__ bool class_out_of_memory(uint32_t tree) const {
+    return sub_trees[tree].looks_empty() && segment_tree.nothing_to_claim();
   }
 
   // the pinned slot a group uses: start at hint % n, step past empty slots
@@ -255,7 +265,7 @@ struct Gallatin {
     int my = (int)(hint % (uint32_t)n);
     const int original = my;
     entry = atom::load(slot_ptr(tree, (uint32_t)my));
-    for (int counter = 0; entry == 0u && my != original - 1;) {
+    for (int counter = 0; (entry == 0u || entry == kSlotBusy) && my != original - 1;) {
       my = (my + 1) % n;
       entry = atom::load(slot_ptr(tree, (uint32_t)my));
       if (++counter >= kPinnedWalk) break;
@@ -284,7 +294,8 @@ struct Gallatin {
     pinned_slot(tree, wave::shfl(hint, leader), index, entry);
     index = wave::shfl(index, leader);
     entry = wave::shfl(entry, leader);
-    const bool has_block = entry != 0u;
+    const bool busy = entry == kSlotBusy;  // being replaced: wait for it
+    const bool has_block = entry != 0u && !busy;
     const uint64_t block = has_block ? entry - 1u : 0;
 
     // exclusive scan of the slice counts inside each group, in lane order
@@ -332,12 +343,13 @@ struct Gallatin {
     const bool replace = ((wave::ballot(should_replace) & group) != 0) || (has_block && c0 > 4095) || stale;
 
     bool refilled = true;
-    const uint64_t rare = wave::ballot(is_leader && (!has_block || replace || completed || give_back > 0)) & looking;
+    const bool empty = entry == 0u;
+    const uint64_t rare = wave::ballot(is_leader && (empty || replace || completed || give_back > 0)) & looking;
     for (uint64_t todo = rare; todo; todo &= todo - 1) {
       if ((int)me == wave::first_lane(todo)) {
         const uint64_t lead_tid = tid;
-        if (!has_block) {
-          refilled = refill_slot(tree, index, lead_tid);
+        if (empty) {
+          refilled = !class_out_of_memory(tree) && refill_slot(tree, index, lead_tid);
         } else {
           if (replace) replace_block(tree, index, block, lead_tid);
           if (completed) recycle_block(block);  // the write-off finished the block
@@ -345,7 +357,8 @@ struct Gallatin {
         }
       }
     }
-    if (!has_block && wave::shfl((uint32_t)refilled, leader) == 0u) {  // class out of memory
+    if (wave::ballot(busy) & looking) wave::backoff(8);
+    if (empty && wave::shfl((uint32_t)refilled, leader) == 0u) {  // class out of memory
       done = true;
       oom = true;
       result = utils::kFail;

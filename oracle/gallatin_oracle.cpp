@@ -18,9 +18,9 @@
  *      an invalid free instead of a huge segment free (gallatin.cuh:1394).
  *  Q6  a pinned slot left empty by an out-of-memory replace_block
  *      (gallatin.cuh:1089-1116) is refilled by the next group that lands on
- *      it, instead of staying empty for the life of the allocator; a racing
- *      refill hands its block back with free_block instead of trapping
- *      (gallatin.cuh:1118-1131).
+ *      it, instead of staying empty for the life of the allocator. (The GPU
+ *      marks a slot busy while one lane replaces or refills it, so no refill
+ *      can race a replacement; a single lane never observes the marker.)
  *  Q13 a group whose leftover correction (block.cuh:267-281) completes a
  *      block's free count (possible only when the block's last slices go to
  *      a multi-slice request that straddles slice 4095) returns the block to
