@@ -67,7 +67,27 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads-log2", type=int, default=22, help="oracle sample size (threads per round)")
+    ap.add_argument(
+        "--copy-calibration",
+        type=int,
+        default=0,
+        metavar="MIB",
+        help="only run 3 plain device copies of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE counters) and exit",
+    )
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -90,10 +110,14 @@ def main():
         """One step: reset, every round (malloc kernel + free kernel), drain."""
         g.mixed_reset(threads, rounds, seed)
         total = None
+        before = g.stats()
         for r in range(rounds):
             res = g.mixed_round(r)
+            now = g.stats()
+            delta = {k: now[k] - before[k] for k in now if now[k] != before[k]}
+            before = now
             log(f"[bench] {tag} round {r}: mallocs {res['mallocs']} frees {res['frees']} misses {res['misses']} "
-                f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f} stats {g.stats()}")
+                f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f} rare paths {delta}")
             total = res if total is None else {k: total[k] + res[k] for k in total}
         res = g.mixed_drain()
         total = {k: total[k] + res[k] for k in total}
@@ -180,6 +204,15 @@ def main():
         },
     }
     g.free_on_device()
+
+    # the device-scope atomic rates that bound the slice path
+    out["atomic_rates_per_s"] = {
+        "distinct_lanes": ga.atomic_rate(ga.ATOMIC_DISTINCT, 1 << 22, 64),
+        "wave_hot_4_words": ga.atomic_rate(ga.ATOMIC_WAVE_HOT, 1 << 22, 16, 4),
+        "wave_hot_128_words": ga.atomic_rate(ga.ATOMIC_WAVE_HOT, 1 << 22, 16, 128),
+        "wave_hot_65536_words": ga.atomic_rate(ga.ATOMIC_WAVE_HOT, 1 << 22, 16, 65536),
+    }
+    log(f"[bench] atomic rates {out['atomic_rates_per_s']}")
 
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")

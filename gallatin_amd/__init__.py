@@ -126,11 +126,22 @@ class Gallatin:
         _capi.check("gal_error_sites", _lib.gal_error_sites(self._h, buf, 14))
         return {name: buf[i] for i, name in enumerate(self.ERROR_SITES) if buf[i]}
 
-    STAT_NAMES = ("replace", "refill", "request", "request_retry", "attach", "release", "recycle")
+    STAT_NAMES = (
+        "replace",
+        "refill",
+        "request",
+        "request_retry",
+        "attach",
+        "release",
+        "recycle",
+        "replace_ticks16",
+        "refill_ticks16",
+    )
 
     def stats(self):
-        buf = (ctypes.c_uint32 * 7)()
-        _capi.check("gal_stats", _lib.gal_stats(self._h, buf, 7))
+        n = len(self.STAT_NAMES)
+        buf = (ctypes.c_uint32 * n)()
+        _capi.check("gal_stats", _lib.gal_stats(self._h, buf, n))
         return {name: buf[i] for i, name in enumerate(self.STAT_NAMES)}
 
     def snapshot(self):
@@ -232,3 +243,15 @@ def global_smoke(n_threads, size):
     m, t = ctypes.c_uint64(), ctypes.c_uint64()
     _capi.check("gal_global_smoke", _lib.gal_global_smoke(n_threads, size, ctypes.byref(m), ctypes.byref(t)))
     return m.value, t.value
+
+
+# ---- device atomic rates (the bound of the slice path; no reference counterpart)
+ATOMIC_DISTINCT = 0
+ATOMIC_WAVE_HOT = 1
+
+
+def atomic_rate(mode, threads, iters, addresses=1):
+    """Returned agent-scope 32-bit atomic adds per second (see gal_atomic_rate)."""
+    out = ctypes.c_double()
+    _capi.check("gal_atomic_rate", _lib.gal_atomic_rate(mode, threads, iters, addresses, ctypes.byref(out)))
+    return out.value

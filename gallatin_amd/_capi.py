@@ -101,6 +101,7 @@ _SIGNATURES = {
     ),
     "gal_run_mixed": (ctypes.c_int, [vp, u64, u32, u64, u32, ctypes.POINTER(GalRunResult)]),
     "gal_read_pool": (ctypes.c_int, [vp, u64, vp, u64]),
+    "gal_atomic_rate": (ctypes.c_int, [u32, u64, u32, u32, ctypes.POINTER(ctypes.c_double)]),
     "gal_latency_samples": (
         ctypes.c_int,
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],

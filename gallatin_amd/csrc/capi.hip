@@ -137,6 +137,25 @@ __global__ void plain_kernel(A* a, uint64_t n, uint64_t size, uint64_t seed, uin
   count(c, got, got, miss, bad);
 }
 
+// Agent-scope atomic rate probe. Mode GAL_ATOMIC_DISTINCT: every lane adds
+// to its own word. Mode GAL_ATOMIC_WAVE_HOT: the first lane of every wave adds
+// to one of `addresses` words (one per 128-B line), the access pattern of a
+// wave-aggregated claim on a pinned block's counter. Returned values are
+// consumed so that each add is a full round trip, as in the allocator.
+__global__ void atomic_rate_kernel(uint32_t* words, uint64_t n, uint32_t iters, uint32_t mode, uint32_t addresses,
+                                   uint32_t* sink) {
+  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  if (tid >= n) return;
+  uint32_t acc = 0;
+  if (mode == GAL_ATOMIC_DISTINCT) {
+    for (uint32_t i = 0; i < iters; i++) acc += atom::add(&words[tid], 1u);
+  } else if ((tid & 63) == 0) {
+    uint32_t* w = &words[(uint64_t)((tid >> 6) % addresses) * 32];
+    for (uint32_t i = 0; i < iters; i++) acc += atom::add(w, 64u);
+  }
+  if (acc == 0xFFFFFFFFu) *sink = acc;  // keeps the results live
+}
+
 // device wall clock (constant rate, see hipDeviceAttributeWallClockRate)
 __device__ inline uint64_t wall_ticks() { return __builtin_amdgcn_s_memrealtime(); }
 
@@ -227,7 +246,8 @@ struct impl {
     const uint64_t nseg = i.num_segments, nblocks = nseg * A::kBlocksPerSegment;
     uint64_t bytes = words * 8 + ((nseg * 2 + 7) / 8) * 8 + nblocks * 12 + nseg * 16 + i.total_slots * 4;
     i.snapshot_bytes = ((bytes + 7) / 8) * 8;
-    i.metadata_bytes = words * 8 + nseg * 4 + nblocks * 12 + nseg * 16 + i.total_slots * 4 + sizeof(A);
+    i.metadata_bytes =
+        words * 8 + nseg * 4 + nblocks * (sizeof(ga::Block) + 4) + nseg * 16 + i.total_slots * 4 + sizeof(A);
     h->control = host.control;
     return GAL_OK;
   }
@@ -264,7 +284,9 @@ struct impl {
     uint64_t pad = ((nseg * 2 + 7) / 8) * 8 - nseg * 2;
     std::memset(p, 0, pad);
     p += pad;
-    if (hipMemcpy(p, host.table.blocks, nblocks * 8, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    // the packed {malloc, free} pairs of the padded block records
+    if (hipMemcpy2D(p, 8, host.table.blocks, sizeof(ga::Block), 8, nblocks, hipMemcpyDeviceToHost) != hipSuccess)
+      return GAL_ERR_HIP;
     p += nblocks * 8;
     if (hipMemcpy(p, host.table.queues, nblocks * 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
     p += nblocks * 4;
@@ -696,6 +718,40 @@ int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n) {
   return hipMemcpy(dst, (const char*)h->info.pool_device_ptr + off, n, hipMemcpyDeviceToHost) == hipSuccess
              ? GAL_OK
              : GAL_ERR_HIP;
+}
+
+int gal_atomic_rate(uint32_t mode, uint64_t threads, uint32_t iters, uint32_t addresses, double* ops_per_s) {
+  if (!ops_per_s || threads == 0 || iters == 0 || mode > GAL_ATOMIC_WAVE_HOT) return GAL_ERR_INVALID;
+  if (mode == GAL_ATOMIC_WAVE_HOT && addresses == 0) return GAL_ERR_INVALID;
+  const uint64_t words = mode == GAL_ATOMIC_DISTINCT ? threads : (uint64_t)addresses * 32;
+  uint32_t* d = nullptr;
+  uint32_t* sink = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = GAL_ERR_HIP;
+  if (hipMalloc(&d, words * sizeof(uint32_t)) == hipSuccess && hipMalloc(&sink, sizeof(uint32_t)) == hipSuccess &&
+      hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+      hipEventCreate(&e1) == hipSuccess && hipMemsetAsync(d, 0, words * sizeof(uint32_t), st) == hipSuccess) {
+    const unsigned grid = blocks_for(threads, 256);
+    atomic_rate_kernel<<<grid, 256, 0, st>>>(d, threads, 1, mode, addresses, sink);  // warm-up
+    (void)hipEventRecord(e0, st);
+    atomic_rate_kernel<<<grid, 256, 0, st>>>(d, threads, iters, mode, addresses, sink);
+    (void)hipEventRecord(e1, st);
+    float ms = 0.f;
+    if (hipGetLastError() == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f) {
+      const double ops =
+          mode == GAL_ATOMIC_DISTINCT ? (double)threads * iters : (double)((threads + 63) / 64) * iters;
+      *ops_per_s = ops / (ms * 1e-3);
+      rc = GAL_OK;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (sink) (void)hipFree(sink);
+  if (d) (void)hipFree(d);
+  return rc;
 }
 
 int gal_init_global_allocator(uint64_t num_bytes, uint64_t seed, int print_info) {

@@ -124,9 +124,24 @@ struct Gallatin {
     return (int64_t)segment;
   }
 
+  // The reference's eager step after a segment's last block is granted
+  // (:1595-1617): attach a fresh segment to the class. The grant's caller
+  // runs it once the block is published, so that a pinned slot is not held
+  // busy for it; with one lane the state after the call is the same.
+  __device__ __noinline__ void gather_after_last(uint32_t tree) {
+    if (acquire_tree_lock(tree)) {
+      gather_new_segment(tree);
+      release_tree_lock(tree);
+    }
+    atom::fence();
+  }
+
   // find a block for class `tree`, attaching a segment when the class has
-  // none with free blocks (reference :1525-1631); tid seeds the hashed search
-  __device__ __noinline__ int64_t request_new_block_from_tree(uint32_t tree, uint64_t tid) {
+  // none with free blocks (reference :1525-1631); tid seeds the hashed
+  // search. `last` reports that the block was its segment's last, after
+  // which the caller runs gather_after_last.
+  __device__ __noinline__ int64_t request_new_block_from_tree(uint32_t tree, uint64_t tid, bool& last) {
+    last = false;
     int oom = 0;
     // Like the reference, only passes that fail for lack of memory count
     // toward giving up; passes that lose a race (the class lock is held, the
@@ -159,15 +174,16 @@ struct Gallatin {
         }
       }
       atom::fence_acquire();
-      bool last = false;
-      int64_t block = table.get_block(segment, tree, last, control);
-      if (last) {
+      bool was_last = false;
+      int64_t block = table.get_block(segment, tree, was_last, control);
+      if (was_last) {
         sub_trees[tree].remove(segment);
-        if (acquire_tree_lock(tree)) {
-          gather_new_segment(tree);
-          release_tree_lock(tree);
+        if (block < 0) {
+          gather_after_last(tree);
+        } else {
+          last = true;
+          return block;
         }
-        atom::fence();
       }
       if (block >= 0) return block;
     }
@@ -214,7 +230,7 @@ struct Gallatin {
 
   // reset the free count of a block whose 4096 slices are all back and hand
   // the block to its segment (reference block.cuh:200-212 + free_block)
-  __device__ void recycle_block(uint64_t block) {
+  __device__ __noinline__ void recycle_block(uint64_t block) {
     // the reset's old value is read: the reset must be performed before the
     // block is queued, or a late reset would wipe its next owner's count
     const uint32_t prev = atom::exch(&table.blocks[block].free_counter, 0u);
@@ -230,11 +246,15 @@ struct Gallatin {
   __device__ __noinline__ bool replace_block(uint32_t tree, uint32_t index, uint64_t block, uint64_t tid) {
     uint32_t* s = slot_ptr(tree, index);
     if (atom::cas(s, (uint32_t)(block + 1), kSlotBusy) != (uint32_t)(block + 1)) return true;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatReplace);
     atom::fence();
-    const int64_t fresh = request_new_block_from_tree(tree, tid);
+    bool last = false;
+    const int64_t fresh = request_new_block_from_tree(tree, tid, last);
     atom::fence_release();  // the block's counter reset precedes its publication
     atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
+    count_stat(control, kStatReplaceTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
+    if (last) gather_after_last(tree);
     return fresh >= 0;
   }
 
@@ -243,10 +263,14 @@ struct Gallatin {
   __device__ __noinline__ bool refill_slot(uint32_t tree, uint32_t index, uint64_t tid) {
     uint32_t* s = slot_ptr(tree, index);
     if (atom::cas(s, 0u, kSlotBusy) != 0u) return true;  // another group is on it
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatRefill);
-    const int64_t fresh = request_new_block_from_tree(tree, tid);
+    bool last = false;
+    const int64_t fresh = request_new_block_from_tree(tree, tid, last);
     atom::fence_release();
     atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
+    count_stat(control, kStatRefillTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
+    if (last) gather_after_last(tree);
     return fresh >= 0;
   }
 
@@ -282,7 +306,7 @@ struct Gallatin {
   // hands each lane its group's old count. Rare per-group work (refilling an
   // empty slot, replacing a spent block, recycling a written-off block,
   // returning slices drawn from a stale block) runs one group at a time.
-  __device__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
+  __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
                              uint64_t& result, bool& done, bool& oom) {
     const uint32_t me = wave::lane_id();
     const uint64_t group = wave::match_groups(looking, tree);
@@ -374,7 +398,7 @@ struct Gallatin {
   // structurizer runs a loop's exit block once, after the loop, for all lanes
   // that left, so per-group work must not sit in one. `oom` is set when the
   // failure is a confirmed out-of-memory, which a further call would repeat.
-  __device__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
+  __device__ __forceinline__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
                                               bool& oom) {
     uint64_t result = utils::kFail;
     bool done = false;
@@ -388,14 +412,16 @@ struct Gallatin {
   }
 
   // whole-block request (reference :828-861 + Block::malloc_fill_block)
-  __device__ uint64_t malloc_block_allocation(uint32_t tree, uint64_t tid) {
-    int64_t block = request_new_block_from_tree(tree, tid);
+  __device__ __noinline__ uint64_t malloc_block_allocation(uint32_t tree, uint64_t tid) {
+    bool last = false;
+    int64_t block = request_new_block_from_tree(tree, tid, last);
     if (block < 0) return utils::kFail;
     Block* b = &table.blocks[block];
     uint32_t old_merged = atom::add(&b->malloc_counter, kSlicesPerBlock);
     uint32_t old_count = old_merged & kBlockCountMask;
     if ((old_merged >> kBlockTreeShift) != tree) count_error(kErrFillClass);
     atom::add(&b->free_counter, 4095u - old_count);
+    if (last) gather_after_last(tree);
     if (old_count != 0) {
       count_error(kErrFillCount);
       free_slices((uint64_t)block, 1u);
@@ -405,7 +431,7 @@ struct Gallatin {
   }
 
   // contiguous segments under the segment lock (reference :790-826)
-  __device__ uint64_t malloc_segment_allocation(uint64_t n) {
+  __device__ __noinline__ uint64_t malloc_segment_allocation(uint64_t n) {
     for (int spin = 0; !acquire_tree_lock(kNumTrees); spin++) {
       if (spin > (1 << 22)) return utils::kFail;
       __builtin_amdgcn_s_sleep(1);
@@ -426,38 +452,41 @@ struct Gallatin {
     return table.memory + (offset / per) * bytes_per_segment + (offset % per) * table_type::tree_alloc_size(tree);
   }
 
+  // Block and segment requests, one lane of the calling lanes at a time.
+  // Kept out of line: the slice path inlined into a kernel then saves no
+  // registers for it.
+  __device__ __noinline__ void* malloc_large(uint64_t size, uint64_t tid, int block_tree) {
+    const bool is_block = (uint32_t)block_tree < kNumTrees;
+    const uint64_t n = is_block ? 0 : (size - 1) / bytes_per_segment + 1;
+    void* out = nullptr;
+    for (uint64_t pending = wave::active_mask(); pending; pending &= pending - 1) {
+      if ((int)wave::lane_id() == wave::first_lane(pending)) {
+        uint64_t offset = utils::kFail;
+        if (is_block) {
+          for (int a = 0; a < kBlockAttempts && offset == utils::kFail; a++)
+            offset = malloc_block_allocation((uint32_t)block_tree, tid);
+          if (offset != utils::kFail) out = offset_to_allocation(offset, (uint32_t)block_tree);
+        } else if (n + kNumTrees + 1 < 0xFFFFull) {
+          for (int a = 0; a < kSegmentAttempts && offset == utils::kFail; a++)
+            offset = malloc_segment_allocation(n);
+          if (offset != utils::kFail) out = offset_to_allocation(offset, 0);
+        }
+      }
+    }
+    return out;
+  }
+
   // --------------------------------------------------------------- malloc
   // malloc with the slot selector and hash id given explicitly; malloc()
   // passes the hardware CU id and the global thread id.
-  __device__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint) {
+  __device__ __forceinline__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint) {
     uint32_t tree = get_tree_id_from_size(size);
     uint32_t count = 1;
     if (tree >= kNumTrees) {
       const int block_tree = utils::get_first_bit_bigger(size) - kSmallestTreeBits;
-      if (block_tree < 0) {
-        count = 1u << (tree - (kNumTrees - 1));
-        tree = kNumTrees - 1;
-      } else {
-        // block and segment requests: one lane of the wave at a time
-        const bool is_block = (uint32_t)block_tree < kNumTrees;
-        const uint64_t n = is_block ? 0 : (size - 1) / bytes_per_segment + 1;
-        void* out = nullptr;
-        for (uint64_t pending = wave::active_mask(); pending; pending &= pending - 1) {
-          if ((int)wave::lane_id() == wave::first_lane(pending)) {
-            uint64_t offset = utils::kFail;
-            if (is_block) {
-              for (int a = 0; a < kBlockAttempts && offset == utils::kFail; a++)
-                offset = malloc_block_allocation((uint32_t)block_tree, tid);
-              if (offset != utils::kFail) out = offset_to_allocation(offset, (uint32_t)block_tree);
-            } else if (n + kNumTrees + 1 < 0xFFFFull) {
-              for (int a = 0; a < kSegmentAttempts && offset == utils::kFail; a++)
-                offset = malloc_segment_allocation(n);
-              if (offset != utils::kFail) out = offset_to_allocation(offset, 0);
-            }
-          }
-        }
-        return out;
-      }
+      if (block_tree >= 0) return malloc_large(size, tid, block_tree);
+      count = 1u << (tree - (kNumTrees - 1));
+      tree = kNumTrees - 1;
     }
     uint64_t offset = utils::kFail;
     bool oom = false;
@@ -468,7 +497,7 @@ struct Gallatin {
   }
 
   // reference gallatin.cuh:1186
-  __device__ void* malloc(uint64_t size) {
+  __device__ __forceinline__ void* malloc(uint64_t size) {
     const uint64_t tid = (uint64_t)(uint32_t)(threadIdx.x + blockIdx.x * blockDim.x);
     return malloc_at(size, tid, __smid());
   }
@@ -479,7 +508,7 @@ struct Gallatin {
   // out again), then one atomic instruction in which each group's first lane
   // adds the group's size to its block's free counter. A block whose count
   // completes is recycled, one at a time.
-  __device__ void free_offset(uint64_t offset) {
+  __device__ __forceinline__ void free_offset(uint64_t offset) {
     const uint64_t block = offset / kSlicesPerBlock;
     const uint64_t group = wave::match_groups(wave::active_mask(), block);
     const bool is_leader = (int)wave::lane_id() == wave::first_lane(group);
@@ -494,8 +523,17 @@ struct Gallatin {
       if ((int)wave::lane_id() == wave::first_lane(todo)) recycle_block(block);
   }
 
+  // A run of segments (reference gallatin.cuh:1283-1302): release the id
+  // first so that no claimer can find the segment in the tree while the old
+  // id is still set.
+  __device__ __noinline__ void free_segment_run(uint64_t segment, uint32_t tree) {
+    if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
+    atom::fence_release();
+    segment_tree.return_multiple(segment, tree - kNumTrees - 1);
+  }
+
   // reference gallatin.cuh:1378-1461
-  __device__ void free(void* allocation) {
+  __device__ __forceinline__ void free(void* allocation) {
     const uint64_t byte = (uint64_t)((char*)allocation - table.memory);
     const uint64_t segment = byte / bytes_per_segment;
     if (byte >= table.num_segments * bytes_per_segment) {
@@ -508,11 +546,7 @@ struct Gallatin {
       return;
     }
     if (tree > kNumTrees) {
-      // a run of segments: release the id first so that no claimer can find
-      // the segment in the tree while the old id is still set
-      if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
-      atom::fence_release();
-      segment_tree.return_multiple(segment, tree - kNumTrees - 1);
+      free_segment_run(segment, tree);
       return;
     }
     const uint64_t offset =
@@ -531,10 +565,12 @@ struct Gallatin {
   __device__ bool boot_pinned_blocks() {
     for (uint32_t tree = 0; tree < kNumTrees; tree++) {
       for (uint32_t s = 0; s < pinned_slots_of(tree); s++) {
-        int64_t block = request_new_block_from_tree(tree, 0);
+        bool last = false;
+        int64_t block = request_new_block_from_tree(tree, 0, last);
         if (block < 0) return false;
         if (table.read_tree_id((uint64_t)block / kBlocksPerSegment) != tree) return false;
         if (atom::cas(slot_ptr(tree, s), 0u, (uint32_t)(block + 1)) != 0u) return false;
+        if (last) gather_after_last(tree);
       }
     }
     return true;

@@ -130,7 +130,9 @@ int gal_error_sites(gal_allocator* h, uint32_t* counts, uint32_t cap);
  * pinned blocks swapped out; empty pinned slots refilled; block requests;
  * block-request passes that lost a race and retried; segments attached to a
  * class; segments released to the segment tree; blocks recycled after all
- * their slices came back. Writes min(cap, 7) counts. */
+ * their slices came back; device wall-clock ticks / 16 during which pinned
+ * slots were held busy for replacement and for refill (summed over the
+ * slots). Writes min(cap, 9) counts. */
 int gal_stats(gal_allocator* h, uint32_t* counts, uint32_t cap);
 
 /* Global allocator (Gallatin<16 MiB, 16, 4096> singleton).
@@ -191,6 +193,17 @@ int gal_latency_samples(gal_allocator* h, uint32_t* ticks, uint64_t cap, uint64_
 
 /* Copy n bytes of the pool starting at byte offset off to host memory. */
 int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n);
+
+/* Agent-scope 32-bit atomic add rate of this GPU, in returned adds per
+ * second. GAL_ATOMIC_DISTINCT: `threads` lanes each add `iters` times to
+ * their own word. GAL_ATOMIC_WAVE_HOT: the first lane of each wave of
+ * `threads` adds `iters` times to one of `addresses` words, the pattern of a
+ * wave-aggregated claim on a pinned block's counter. No counterpart in the
+ * reference: it measures the rate that bounds the slice path, for the
+ * roofline. Allocates and frees its own buffer. */
+#define GAL_ATOMIC_DISTINCT 0u
+#define GAL_ATOMIC_WAVE_HOT 1u
+int gal_atomic_rate(uint32_t mode, uint64_t threads, uint32_t iters, uint32_t addresses, double* ops_per_s);
 
 #ifdef __cplusplus
 }
