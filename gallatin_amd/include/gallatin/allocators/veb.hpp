@@ -125,19 +125,51 @@ struct veb_tree {
   // nothing malloc_first could claim: no item 0 and no successor of 0
   __device__ bool nothing_to_claim() const { return !query(0) && successor_thorough(0) == utils::kFail; }
 
-  // hashed start positions (reference veb.cuh:946-977); tid is the caller's
-  // global thread id
+  // Highest item reachable through the summaries: descend along the highest
+  // set bit of each layer; a summary bit whose word below is empty is
+  // stepped over by backing up and continuing below it.
+  __device__ uint64_t max_item() const {
+    int layer = 0;
+    uint64_t high = 0;
+    int below = 64;  // only bits under `below` of word `high` are candidates
+    for (int step = 0; step < kSuccessorRestarts; step++) {
+      const uint64_t w = load_word(layer, high) & utils::bitmask((uint64_t)below);
+      if (w == 0) {
+        if (layer == 0) return utils::kFail;
+        below = (int)(high & 63);
+        high >>= 6;
+        layer--;
+        continue;
+      }
+      const uint64_t b = 63 - utils::clz64(w);
+      if (layer == num_layers - 1) return (high << 6) + b;
+      high = (high << 6) + b;
+      layer++;
+      below = 64;
+    }
+    return utils::kFail;
+  }
+
+  // Hashed start positions (reference veb.cuh:946-977); tid is the caller's
+  // global thread id. An attempt succeeds exactly when some item lies at or
+  // above its start, so starts above the highest item are skipped without
+  // touching memory; the attempts that remain are the reference's. With the
+  // tree unchanged during the call the result is the reference's; a sparse
+  // tree no longer costs a failed successor walk per hashed start.
   __device__ uint64_t find_random_valid_index(uint64_t tid) const {
     if (looks_empty()) return utils::kFail;
+    const uint64_t top = max_item();
+    if (top == utils::kFail) return utils::kFail;
     uint64_t hash1 = utils::murmur64a(tid, seed);
     uint64_t hash2 = utils::murmur64a(tid, hash1);
     for (int attempts = 0; attempts < kVebHashedAttempts; attempts++) {
       uint64_t start = (hash1 + (uint64_t)attempts * hash2) % total_universe;
+      if (start > top) continue;
       if (query(start)) return start;
       uint64_t index = successor_thorough(start);
       if (index != utils::kFail) return index;
     }
-    return successor_thorough(0);
+    return top == 0 ? utils::kFail : successor_thorough(0);
   }
 
   // lowest free item, claimed (reference veb.cuh:979-991)
