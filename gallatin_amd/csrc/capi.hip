@@ -181,21 +181,75 @@ __global__ void mixed_malloc_kernel(A* a, uint64_t* live, uint64_t base, uint64_
   count(c, got, false, miss, bad);
 }
 
+// Each wave scans 64 * kFreeScanIters consecutive entries of the live array
+// and packs the pointers it frees (live, and picked by the round's coin)
+// into full 64-lane batches in LDS before calling free, so that every free
+// call runs with a full wave (one release fence per 64 frees) however sparse
+// the live array has become.
+constexpr int kFreeScanIters = 16;
+constexpr int kFreeWavesPerBlock = 4;
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class A>
-__global__ void mixed_free_kernel(A* a, uint64_t* live, uint64_t n_live, uint32_t round, uint64_t seed,
-                                  uint32_t verify, bool all, run_counters* c) {
-  const uint64_t j = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
-  bool freed = false, bad = false;
-  if (j < n_live) {
-    uint64_t* p = (uint64_t*)live[j];
-    if (p && (all || gw::mixed_coin(seed, round, j))) {
-      bad = !tag_out(p, gw::payload_tag(seed, j), verify);
+__global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
+    mixed_free_kernel(A* a, uint64_t* live, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify, bool all,
+                      run_counters* c) {
+  __shared__ uint64_t pend_p[kFreeWavesPerBlock][128];
+  __shared__ uint64_t pend_j[kFreeWavesPerBlock][128];
+  const uint32_t lane = wave::lane_id();
+  const uint32_t w = threadIdx.x >> 6;
+  const uint64_t wave_index = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t base = wave_index * 64 * kFreeScanIters;
+  uint32_t pending = 0;  // wave-uniform count of packed entries
+  uint64_t freed = 0, bad = 0;
+  auto flush = [&](uint32_t k) {
+    bool b = false;
+    if (lane < k) {
+      uint64_t* p = (uint64_t*)pend_p[w][lane];
+      const uint64_t j = pend_j[w][lane];
+      b = !tag_out(p, gw::payload_tag(seed, j), verify);
       a->free(p);
       live[j] = 0;
-      freed = true;
+    }
+    freed += k;
+    bad += wave::popc(wave::ballot(b));
+  };
+  for (int it = 0; it < kFreeScanIters; it++) {
+    const uint64_t j = base + (uint64_t)it * 64 + lane;
+    uint64_t p = j < n_live ? live[j] : 0;
+    const bool pick = p != 0 && (all || gw::mixed_coin(seed, round, j));
+    const uint64_t m = wave::ballot(pick);
+    if (pick) {
+      const uint32_t pos = pending + wave::popc(m & wave::lanemask_lt());
+      pend_p[w][pos] = p;
+      pend_j[w][pos] = j;
+    }
+    pending += wave::popc(m);
+    wave_lds_sync();
+    if (pending >= 64) {
+      flush(64);
+      pending -= 64;
+      uint64_t mp = 0, mj = 0;
+      if (lane < pending) {
+        mp = pend_p[w][64 + lane];
+        mj = pend_j[w][64 + lane];
+      }
+      wave_lds_sync();
+      if (lane < pending) {
+        pend_p[w][lane] = mp;
+        pend_j[w][lane] = mj;
+      }
+      wave_lds_sync();
     }
   }
-  count(c, false, freed, false, bad);
+  if (pending > 0) flush(pending);
+  if (lane == 0 && freed) atom::add(&c->frees, (unsigned long long)freed);
+  if (lane == 0 && bad) atom::add(&c->tag_errors, (unsigned long long)bad);
 }
 
 __global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) {
@@ -315,8 +369,9 @@ struct impl {
   static void launch_mixed_free(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
                                 bool all) {
     if (n_live == 0) return;
-    mixed_free_kernel<A><<<blocks_for(n_live, 256), 256, 0, h->stream>>>((A*)h->dev, h->d_live, n_live, round, seed,
-                                                                         verify, all, h->d_counters);
+    const uint64_t waves = (n_live + 64 * kFreeScanIters - 1) / (64 * kFreeScanIters);
+    mixed_free_kernel<A><<<(unsigned)((waves + kFreeWavesPerBlock - 1) / kFreeWavesPerBlock), 64 * kFreeWavesPerBlock,
+                           0, h->stream>>>((A*)h->dev, h->d_live, n_live, round, seed, verify, all, h->d_counters);
   }
   static void print(gal_allocator* h) { ((A*)h->dev)->print_info(); }
 
