@@ -47,6 +47,12 @@ constexpr uint32_t kPinnedSlotsClass0 = 128;
 // + 1, or kSlotBusy while one lane replaces or refills it
 constexpr uint32_t kSlotBusy = 0xFFFFFFFFu;
 constexpr uint32_t kPinnedSlotsMin = 4;     // MIN_PINNED_CUTOFF
+// Classes with fewer than kSlotsUnderLoad pinned slots get overflow slots up
+// to that count. A group uses them only when every pinned slot of its class
+// is busy being replaced, which a single lane never sees; under load they
+// let replacements of one class overlap instead of stalling every wave that
+// has a lane of that class.
+constexpr uint32_t kSlotsUnderLoad = 32;
 
 __host__ __device__ constexpr int ilog2(uint64_t x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 __host__ __device__ constexpr bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
@@ -56,6 +62,11 @@ __host__ __device__ constexpr bool is_pow2(uint64_t x) { return x && !(x & (x - 
 __host__ __device__ constexpr uint32_t pinned_slots_of(uint32_t c) {
   return (kPinnedSlotsClass0 >> (c > 31 ? 31 : c)) < kPinnedSlotsMin ? kPinnedSlotsMin : (kPinnedSlotsClass0 >> c);
 }
+__host__ __device__ constexpr uint32_t overflow_slots_of(uint32_t c) {
+  return pinned_slots_of(c) < kSlotsUnderLoad ? kSlotsUnderLoad - pinned_slots_of(c) : 0u;
+}
+// slot region of class c: its pinned slots, then its overflow slots
+__host__ __device__ constexpr uint32_t slot_region_of(uint32_t c) { return pinned_slots_of(c) + overflow_slots_of(c); }
 
 template <uint64_t bytes_per_segment, uint64_t smallest, uint64_t biggest>
 struct Gallatin {
@@ -74,7 +85,7 @@ struct Gallatin {
 
   static constexpr uint32_t total_slots() {
     uint32_t t = 0;
-    for (uint32_t c = 0; c < kNumTrees; c++) t += pinned_slots_of(c);
+    for (uint32_t c = 0; c < kNumTrees; c++) t += slot_region_of(c);
     return t;
   }
 
@@ -284,16 +295,23 @@ struct Gallatin {
   // the pinned slot a group uses: start at hint % n, step past empty slots
   // (at most 30 steps, reference shared_block_storage.cuh:119-145); returns
   // the slot's index and the entry read there
+  // The reference's walk over the pinned slots. Once a busy slot has been
+  // seen the walk ends after one cycle, reporting busy (the caller then
+  // tries the overflow slots); a single lane never sees a busy slot.
   __device__ void pinned_slot(uint32_t tree, uint32_t hint, uint32_t& index, uint32_t& entry) const {
     const int n = (int)pinned_slots_of(tree);
     int my = (int)(hint % (uint32_t)n);
     const int original = my;
     entry = atom::load(slot_ptr(tree, (uint32_t)my));
+    bool saw_busy = entry == kSlotBusy;
     for (int counter = 0; (entry == 0u || entry == kSlotBusy) && my != original - 1;) {
+      if (saw_busy && counter + 1 >= n) break;
       my = (my + 1) % n;
       entry = atom::load(slot_ptr(tree, (uint32_t)my));
+      saw_busy |= entry == kSlotBusy;
       if (++counter >= kPinnedWalk) break;
     }
+    if (saw_busy && (entry == 0u || entry == kSlotBusy)) entry = kSlotBusy;
     index = (uint32_t)my;
   }
 
@@ -307,7 +325,7 @@ struct Gallatin {
   // empty slot, replacing a spent block, recycling a written-off block,
   // returning slices drawn from a stale block) runs one group at a time.
   __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
-                             uint64_t& result, bool& done, bool& oom) {
+                                             uint32_t attempt, uint64_t& result, bool& done, bool& oom) {
     const uint32_t me = wave::lane_id();
     const uint64_t group = wave::match_groups(looking, tree);
     const int leader = wave::first_lane(group);
@@ -315,9 +333,36 @@ struct Gallatin {
     const bool is_leader = (int)me == leader;
 
     uint32_t index, entry;
-    pinned_slot(tree, wave::shfl(hint, leader), index, entry);
+    const uint32_t lead_hint = wave::shfl(hint, leader);
+    pinned_slot(tree, lead_hint, index, entry);
     index = wave::shfl(index, leader);
     entry = wave::shfl(entry, leader);
+    // Every pinned slot of the class busy: the group's lanes each look at
+    // one overflow slot, from a start that moves with the hint and the pass,
+    // and take the first holding a block, else the first empty one (which
+    // the group then fills), else stay busy.
+    if (wave::ballot(entry == kSlotBusy && overflow_slots_of(tree) > 0) & looking) {
+      const uint32_t k = overflow_slots_of(tree);
+      const bool want = entry == kSlotBusy && k > 0;
+      const uint32_t rank = wave::popc(group & wave::lanemask_lt());
+      const bool look = want && rank < k;
+      uint32_t my_index = 0, my_entry = kSlotBusy;
+      if (look) {
+        const uint32_t start = (lead_hint * 0x9E3779B1u + attempt * 0x85EBCA6Bu) % k;
+        my_index = pinned_slots_of(tree) + (start + rank) % k;
+        my_entry = atom::load(slot_ptr(tree, my_index));
+      }
+      const uint64_t with_block = wave::ballot(look && my_entry != 0u && my_entry != kSlotBusy) & group;
+      const uint64_t empty_one = wave::ballot(look && my_entry == 0u) & group;
+      const uint64_t pick = with_block ? with_block : empty_one;
+      const int src = (want && pick) ? wave::first_lane(pick) : (int)me;
+      const uint32_t picked_index = wave::shfl(my_index, src);
+      const uint32_t picked_entry = wave::shfl(my_entry, src);
+      if (want && pick) {
+        index = picked_index;
+        entry = picked_entry;
+      }
+    }
     const bool busy = entry == kSlotBusy;  // being replaced: wait for it
     const bool has_block = entry != 0u && !busy;
     const uint64_t block = has_block ? entry - 1u : 0;
@@ -405,7 +450,8 @@ struct Gallatin {
     oom = false;
     uint64_t looking = wave::active_mask();
     for (int attempt = 0; attempt < kSliceAttempts && looking; attempt++) {
-      if ((looking >> wave::lane_id()) & 1ull) claim_pass(looking, tree, count, tid, hint, result, done, oom);
+      if ((looking >> wave::lane_id()) & 1ull)
+        claim_pass(looking, tree, count, tid, hint, (uint32_t)attempt, result, done, oom);
       looking &= ~wave::ballot(done);
     }
     return result;
@@ -711,7 +757,7 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_b
   uint32_t off = 0;
   for (uint32_t c = 0; c < kNumTrees; c++) {
     host.slot_offset[c] = off;
-    off += pinned_slots_of(c);
+    off += slot_region_of(c);
   }
   if (hipMemset(table_mem, 0, table_bytes) != hipSuccess) return fail();
   gallatin_fill_u32<<<detail::grid_for(nseg), 256>>>(host.table.chunk_ids, nseg, kFreeChunk);

@@ -819,9 +819,15 @@ struct Alloc {
     return true;
   }
 
+  // The GPU gives a class with fewer than 32 pinned slots overflow slots up
+  // to 32, used only when all its pinned slots are busy being replaced by
+  // other lanes; with one lane they stay empty. The snapshot lists each
+  // class's pinned slots followed by its (empty) overflow slots.
+  static uint64_t overflow_slots(uint64_t pinned) { return pinned < 32 ? 32 - pinned : 0; }
+
   uint64_t total_slots() const {
     uint64_t t = 0;
-    for (auto& s : slots) t += s.size();
+    for (auto& s : slots) t += s.size() + overflow_slots(s.size());
     return t;
   }
 
@@ -864,7 +870,11 @@ struct Alloc {
       put(&table.q_free[s], 4);
       put(&table.q_final[s], 4);
     }
-    for (auto& s : slots) put(s.data(), s.size() * 4);
+    for (auto& s : slots) {
+      put(s.data(), s.size() * 4);
+      const std::vector<uint32_t> overflow(overflow_slots(s.size()), 0u);
+      if (!overflow.empty()) put(overflow.data(), overflow.size() * 4);
+    }
     size_t used = (size_t)(p - out);
     size_t total = snapshot_bytes();
     if (total > used) std::memset(p, 0, total - used);
