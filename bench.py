@@ -27,6 +27,12 @@ HBM_PEAK_GBS = 8000.0
 # Malloc kernel share: 8 + 8 + 8 = 24 B; free kernel share: 8 + 2 + 8 = 18 B.
 BYTES_MALLOC_SIDE = 24
 BYTES_FREE_SIDE = 18
+# HBM bytes per launch of the workload kernels at the default workload, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch, averaged over a
+# step) scaled by what the counters read for a 1 GiB device copy (fetch x2.0,
+# write x1.0): profiles/mixed_slices_8GiB_hbm_traffic.md.
+MEASURED_TRAFFIC_BYTES = {"mixed_malloc_kernel": 212.8e6, "mixed_free_kernel": 395.0e6}
+DEFAULT_WORKLOAD = (8, 22, 10)  # pool GiB, log2 threads per round, rounds
 
 
 def log(msg):
@@ -155,6 +161,8 @@ def main():
     else:
         dom, n_launch, dom_ms, dom_bytes = "mixed_free_kernel", (rounds + 1) * args.steps, free_ms, frees * BYTES_FREE_SIDE
     achieved = (dom_bytes / n_launch) / ((dom_ms / n_launch) * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    default = (args.pool_gib, args.threads_log2, args.rounds) == DEFAULT_WORKLOAD
+    traffic = MEASURED_TRAFFIC_BYTES[dom] if default else None
 
     import numpy as np
 
@@ -200,7 +208,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": dom_bytes / n_launch,
+            "kernel_avg_us": dom_ms / n_launch * 1e3,
         },
     }
     g.free_on_device()
