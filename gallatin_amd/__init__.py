@@ -245,6 +245,75 @@ def global_smoke(n_threads, size):
     return m.value, t.value
 
 
+# ---- batches of uneven requests (no reference counterpart; same device API)
+FORM_PER_THREAD = _capi.GAL_FORM_PER_THREAD
+FORM_BATCHED = _capi.GAL_FORM_BATCHED
+FORM_GRAPH = _capi.GAL_FORM_GRAPH
+
+
+class Batch:
+    """Requests served together: binned by class on the device, then one
+    persistent launch (see gal_batch_* in include/gallatin_amd.h)."""
+
+    def __init__(self, allocator, capacity):
+        h = ctypes.c_void_p()
+        _capi.check("gal_batch_create", _lib.gal_batch_create(allocator._h, capacity, ctypes.byref(h)))
+        self._b = h
+        self.allocator = allocator
+        self.capacity = capacity
+        self.n = 0
+
+    def close(self):
+        if self._b:
+            _capi.check("gal_batch_destroy", _lib.gal_batch_destroy(self._b))
+            self._b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def generate(self, n, seed, size_classes=9):
+        _capi.check("gal_batch_generate", _lib.gal_batch_generate(self._b, n, seed, size_classes))
+        self.n = n
+
+    def set_sizes(self, sizes, seed=0):
+        arr = np.ascontiguousarray(sizes, dtype=np.uint64)
+        p = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        _capi.check("gal_batch_set_sizes", _lib.gal_batch_set_sizes(self._b, p, len(arr), seed))
+        self.n = len(arr)
+
+    def run(self, form, width=64, replays=1, verify=GAL_VERIFY_STORE, run=1):
+        r = _capi.GalRunResult()
+        _capi.check(
+            "gal_batch_run", _lib.gal_batch_run(self._b, form, width, run, replays, verify, ctypes.byref(r))
+        )
+        return r.as_dict()
+
+    def malloc(self, width=64, run=1):
+        r = _capi.GalRunResult()
+        _capi.check("gal_batch_malloc", _lib.gal_batch_malloc(self._b, width, run, ctypes.byref(r)))
+        return r.as_dict()
+
+    def free(self):
+        r = _capi.GalRunResult()
+        _capi.check("gal_batch_free", _lib.gal_batch_free(self._b, ctypes.byref(r)))
+        return r.as_dict()
+
+    def offsets(self):
+        out = np.zeros(max(1, self.n), dtype=np.uint64)
+        p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        _capi.check("gal_batch_offsets", _lib.gal_batch_offsets(self._b, p, self.n))
+        return out[: self.n]
+
+    def order(self):
+        out = np.zeros(max(1, self.n), dtype=np.uint32)
+        p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        _capi.check("gal_batch_order", _lib.gal_batch_order(self._b, p, self.n))
+        return out[: self.n]
+
+
 # ---- device atomic rates (the bound of the slice path; no reference counterpart)
 ATOMIC_DISTINCT = 0
 ATOMIC_WAVE_HOT = 1

@@ -32,6 +32,9 @@ GAL_OP_FREE = 1
 GAL_NULL_OFFSET = (1 << 64) - 1
 GAL_VERIFY_STORE = 0
 GAL_VERIFY_ATOMIC = 1
+GAL_FORM_PER_THREAD = 0
+GAL_FORM_BATCHED = 1
+GAL_FORM_GRAPH = 2
 
 u64 = ctypes.c_uint64
 u32 = ctypes.c_uint32
@@ -102,6 +105,15 @@ _SIGNATURES = {
     "gal_run_mixed": (ctypes.c_int, [vp, u64, u32, u64, u32, ctypes.POINTER(GalRunResult)]),
     "gal_read_pool": (ctypes.c_int, [vp, u64, vp, u64]),
     "gal_atomic_rate": (ctypes.c_int, [u32, u64, u32, u32, ctypes.POINTER(ctypes.c_double)]),
+    "gal_batch_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
+    "gal_batch_destroy": (ctypes.c_int, [vp]),
+    "gal_batch_generate": (ctypes.c_int, [vp, u64, u64, u32]),
+    "gal_batch_set_sizes": (ctypes.c_int, [vp, ctypes.POINTER(u64), u64, u64]),
+    "gal_batch_run": (ctypes.c_int, [vp, u32, u32, u32, u32, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_batch_malloc": (ctypes.c_int, [vp, u32, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_batch_free": (ctypes.c_int, [vp, ctypes.POINTER(GalRunResult)]),
+    "gal_batch_offsets": (ctypes.c_int, [vp, ctypes.POINTER(u64), u64]),
+    "gal_batch_order": (ctypes.c_int, [vp, ctypes.POINTER(u32), u64]),
     "gal_latency_samples": (
         ctypes.c_int,
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],

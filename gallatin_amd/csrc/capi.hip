@@ -54,6 +54,20 @@ struct gal_allocator {
   double tick_ns = 10.0;
 };
 
+// a batch of requests and the device buffers that bin and serve it
+struct gal_batch {
+  gal_allocator* h = nullptr;
+  uint64_t capacity = 0;
+  uint64_t n = 0;            // requests in the batch
+  uint64_t seed = 0;         // tags are drawn from (seed, request)
+  uint64_t* d_sizes = nullptr;
+  uint32_t* d_perm = nullptr;     // binned order of the last binning
+  uint32_t* d_counts = nullptr;   // per binning wave and bin
+  uint32_t* d_offsets = nullptr;  // per binning wave and bin
+  uint32_t* d_totals = nullptr;   // per bin
+  uint64_t** d_ptrs = nullptr;    // results of gal_batch_malloc
+};
+
 namespace {
 
 struct impl_ops {
@@ -69,6 +83,10 @@ struct impl_ops {
   void (*launch_mixed_free)(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
                             bool all);
   void (*print)(gal_allocator* h);
+  // batches (batch.inl)
+  void (*batch_bin)(gal_batch* b);
+  void (*batch_pass)(gal_batch* b, uint32_t mode, uint32_t width_log2, uint32_t run, uint32_t verify);
+  void (*batch_per_thread)(gal_batch* b, uint32_t verify);
 };
 
 // ---------------------------------------------------------------- kernels
@@ -272,6 +290,8 @@ __global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) 
 
 inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
+#include "batch.inl"
+
 // ------------------------------------------------------ per-type impl
 template <uint64_t S, uint64_t M, uint64_t B>
 struct impl {
@@ -338,10 +358,18 @@ struct impl {
     uint64_t pad = ((nseg * 2 + 7) / 8) * 8 - nseg * 2;
     std::memset(p, 0, pad);
     p += pad;
-    // the packed {malloc, free} pairs of the padded block records
-    if (hipMemcpy2D(p, 8, host.table.blocks, sizeof(ga::Block), 8, nblocks, hipMemcpyDeviceToHost) != hipSuccess)
+    // {malloc, free} per block in the reference's packing (class << 20 | count)
+    std::vector<uint64_t> rec(nblocks * 2);
+    if (hipMemcpy2D(rec.data(), 16, host.table.blocks, sizeof(ga::Block), 16, nblocks, hipMemcpyDeviceToHost) !=
+        hipSuccess)
       return GAL_ERR_HIP;
-    p += nblocks * 8;
+    for (uint64_t k = 0; k < nblocks; k++) {
+      const uint64_t m = rec[2 * k];
+      const uint32_t count = ga::counter_count(m) > 0xFFFFFu ? 0xFFFFFu : ga::counter_count(m);
+      const uint32_t pair[2] = {(ga::counter_class(m) << ga::kSnapshotClassShift) | count, (uint32_t)rec[2 * k + 1]};
+      std::memcpy(p, pair, 8);
+      p += 8;
+    }
     if (hipMemcpy(p, host.table.queues, nblocks * 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
     p += nblocks * 4;
     if (hipMemcpy(p, host.table.segs, nseg * 16, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
@@ -375,10 +403,41 @@ struct impl {
   }
   static void print(gal_allocator* h) { ((A*)h->dev)->print_info(); }
 
+  static void batch_bin(gal_batch* b) {
+    const uint64_t waves = (b->n + kBinWaveRequests - 1) / kBinWaveRequests;
+    const unsigned grid = (unsigned)((waves + 3) / 4);
+    bin_count_kernel<A><<<grid, 256, 0, b->h->stream>>>(b->d_sizes, b->n, b->d_counts);
+    bin_scan_kernel<<<kBins, 256, 0, b->h->stream>>>(b->d_counts, waves, b->d_offsets, b->d_totals);
+    bin_scatter_kernel<A><<<grid, 256, 0, b->h->stream>>>(b->d_sizes, b->n, b->d_offsets, b->d_totals, b->d_perm);
+  }
+  // persistent grid: the blocks that fit the GPU at once, no more than the chunks need
+  static unsigned persistent_blocks(uint64_t n) {
+    static int resident = 0;
+    if (resident == 0) {
+      int per_cu = 0, cus = 0, device = 0;
+      if (hipGetDevice(&device) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<A>, 256, 0) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1)
+        per_cu = 1, cus = 256;
+      resident = per_cu * cus;
+    }
+    const uint64_t need = (n + 255) / 256;
+    return (unsigned)(need < (uint64_t)resident ? (need ? need : 1) : (uint64_t)resident);
+  }
+  static void batch_pass(gal_batch* b, uint32_t mode, uint32_t width_log2, uint32_t run, uint32_t verify) {
+    batch_kernel<A><<<persistent_blocks((b->n + run - 1) / run), 256, 0, b->h->stream>>>(
+        (A*)b->h->dev, b->d_sizes, b->d_perm, b->n, b->d_ptrs, mode, width_log2, run, b->seed, verify,
+        b->h->d_counters);
+  }
+  static void batch_per_thread(gal_batch* b, uint32_t verify) {
+    batch_plain_kernel<A><<<blocks_for(b->n, 256), 256, 0, b->h->stream>>>((A*)b->h->dev, b->d_sizes, b->n, b->seed,
+                                                                           verify, b->h->d_counters);
+  }
+
   static constexpr impl_ops ops() {
-    return impl_ops{S,       M,        B,           &create,          &destroy,
-                    &describe, &snapshot, &replay, &launch_plain, &launch_mixed_malloc,
-                    &launch_mixed_free, &print};
+    return impl_ops{S,         M,          B,       &create,       &destroy,           &describe,
+                    &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
+                    &batch_bin, &batch_pass, &batch_per_thread};
   }
 };
 
@@ -773,6 +832,139 @@ int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n) {
   return hipMemcpy(dst, (const char*)h->info.pool_device_ptr + off, n, hipMemcpyDeviceToHost) == hipSuccess
              ? GAL_OK
              : GAL_ERR_HIP;
+}
+
+static int width_log2_of(uint32_t width, uint32_t* out) {
+  for (uint32_t l = 0; l <= 6; l++)
+    if ((1u << l) == width) {
+      *out = l;
+      return GAL_OK;
+    }
+  return GAL_ERR_INVALID;
+}
+
+int gal_batch_destroy(gal_batch* b) {
+  if (!b) return GAL_ERR_INVALID;
+  if (b->h) (void)hipStreamSynchronize(b->h->stream);
+  void* bufs[] = {b->d_sizes, b->d_perm, b->d_counts, b->d_offsets, b->d_totals, b->d_ptrs};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  delete b;
+  return GAL_OK;
+}
+
+int gal_batch_create(gal_allocator* h, uint64_t capacity, gal_batch** out) {
+  if (!h || !out || capacity == 0 || capacity > 0xFFFFFFFFull) return GAL_ERR_INVALID;
+  *out = nullptr;
+  gal_batch* b = new gal_batch();
+  b->h = h;
+  b->capacity = capacity;
+  const uint64_t waves = (capacity + kBinWaveRequests - 1) / kBinWaveRequests;
+  const bool ok = hipMalloc(&b->d_sizes, capacity * 8) == hipSuccess &&
+                  hipMalloc(&b->d_perm, capacity * 4) == hipSuccess &&
+                  hipMalloc(&b->d_counts, waves * kBins * 4) == hipSuccess &&
+                  hipMalloc(&b->d_offsets, waves * kBins * 4) == hipSuccess &&
+                  hipMalloc(&b->d_totals, kBins * 4) == hipSuccess && hipMalloc(&b->d_ptrs, capacity * 8) == hipSuccess &&
+                  hipMemsetAsync(b->d_ptrs, 0, capacity * 8, h->stream) == hipSuccess &&
+                  hipStreamSynchronize(h->stream) == hipSuccess;
+  if (!ok) {
+    gal_batch_destroy(b);
+    return GAL_ERR_OOM;
+  }
+  *out = b;
+  return GAL_OK;
+}
+
+int gal_batch_generate(gal_batch* b, uint64_t n, uint64_t seed, uint32_t size_classes) {
+  if (!b || n == 0 || n > b->capacity || size_classes == 0 || size_classes > 40) return GAL_ERR_INVALID;
+  b->n = n;
+  b->seed = seed;
+  batch_sizes_kernel<<<blocks_for(n, 256), 256, 0, b->h->stream>>>(b->d_sizes, n, seed, size_classes);
+  return hipStreamSynchronize(b->h->stream) == hipSuccess ? GAL_OK : GAL_ERR_HIP;
+}
+
+int gal_batch_set_sizes(gal_batch* b, const uint64_t* sizes, uint64_t n, uint64_t seed) {
+  if (!b || !sizes || n == 0 || n > b->capacity) return GAL_ERR_INVALID;
+  if (hipMemcpy(b->d_sizes, sizes, n * 8, hipMemcpyHostToDevice) != hipSuccess) return GAL_ERR_HIP;
+  b->n = n;
+  b->seed = seed;
+  return GAL_OK;
+}
+
+int gal_batch_run(gal_batch* b, uint32_t form, uint32_t width, uint32_t run, uint32_t replays, uint32_t verify,
+                  gal_run_result* out) {
+  uint32_t wl = 6;
+  if (!b || b->n == 0 || replays == 0 || verify > GAL_VERIFY_ATOMIC || form > GAL_FORM_GRAPH ||
+      width_log2_of(width, &wl) != GAL_OK || run == 0 || run > kMaxRun)
+    return GAL_ERR_INVALID;
+  gal_allocator* h = b->h;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (form == GAL_FORM_GRAPH) {
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return GAL_ERR_HIP;
+    h->ops->batch_bin(b);
+    h->ops->batch_pass(b, kBatchMalloc | kBatchFree, wl, run, verify);
+    if (hipStreamEndCapture(h->stream, &graph) != hipSuccess) return GAL_ERR_HIP;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return GAL_ERR_HIP;
+    }
+  }
+  int rc = timed_run(h, out, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    for (uint32_t r = 0; r < replays; r++) {
+      if (form == GAL_FORM_PER_THREAD) {
+        h->ops->batch_per_thread(b, verify);
+      } else if (form == GAL_FORM_BATCHED) {
+        h->ops->batch_bin(b);
+        h->ops->batch_pass(b, kBatchMalloc | kBatchFree, wl, run, verify);
+      } else if (hipGraphLaunch(exec, h->stream) != hipSuccess) {
+        return GAL_ERR_HIP;
+      }
+    }
+    t.end(0);
+    return GAL_OK;
+  });
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  return rc;
+}
+
+int gal_batch_malloc(gal_batch* b, uint32_t width, uint32_t run, gal_run_result* out) {
+  uint32_t wl = 6;
+  if (!b || b->n == 0 || width_log2_of(width, &wl) != GAL_OK || run == 0 || run > kMaxRun) return GAL_ERR_INVALID;
+  return timed_run(b->h, out, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    b->h->ops->batch_bin(b);
+    b->h->ops->batch_pass(b, kBatchMalloc, wl, run, GAL_VERIFY_STORE);
+    t.end(0);
+    return GAL_OK;
+  });
+}
+
+int gal_batch_free(gal_batch* b, gal_run_result* out) {
+  if (!b || b->n == 0) return GAL_ERR_INVALID;
+  return timed_run(b->h, out, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    b->h->ops->batch_pass(b, kBatchFree, 6, 1, GAL_VERIFY_STORE);
+    t.end(1);
+    return GAL_OK;
+  });
+}
+
+int gal_batch_offsets(gal_batch* b, uint64_t* offsets, uint64_t cap) {
+  if (!b || !offsets || cap < b->n) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(b->h->stream) != hipSuccess) return GAL_ERR_HIP;
+  if (hipMemcpy(offsets, b->d_ptrs, b->n * 8, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  const uint64_t base = b->h->info.pool_device_ptr;
+  for (uint64_t i = 0; i < b->n; i++) offsets[i] = offsets[i] ? offsets[i] - base : GAL_NULL_OFFSET;
+  return GAL_OK;
+}
+
+int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap) {
+  if (!b || !order || cap < b->n) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(b->h->stream) != hipSuccess) return GAL_ERR_HIP;
+  return hipMemcpy(order, b->d_perm, b->n * 4, hipMemcpyDeviceToHost) == hipSuccess ? GAL_OK : GAL_ERR_HIP;
 }
 
 int gal_atomic_rate(uint32_t mode, uint64_t threads, uint32_t iters, uint32_t addresses, double* ops_per_s) {

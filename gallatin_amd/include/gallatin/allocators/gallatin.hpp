@@ -257,6 +257,17 @@ struct Gallatin {
   __device__ __noinline__ bool replace_block(uint32_t tree, uint32_t index, uint64_t block, uint64_t tid) {
     uint32_t* s = slot_ptr(tree, index);
     if (atom::cas(s, (uint32_t)(block + 1), kSlotBusy) != (uint32_t)(block + 1)) return true;
+    // The slot may hold a later incarnation of the same block: between the
+    // claim that found it spent and this swap, the block can be freed,
+    // recycled, granted again and published in this very slot. Only a block
+    // that is spent (4096 slices handed out) or serves another class is
+    // replaced; a live one is put back. With one lane the block is always
+    // the spent one.
+    const uint64_t now = atom::load(&table.blocks[block].malloc_counter);
+    if (counter_class(now) == tree && counter_count(now) < kSlicesPerBlock) {
+      atom::exch(s, (uint32_t)(block + 1));
+      return true;
+    }
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatReplace);
     atom::fence();
@@ -324,10 +335,15 @@ struct Gallatin {
   // hands each lane its group's old count. Rare per-group work (refilling an
   // empty slot, replacing a spent block, recycling a written-off block,
   // returning slices drawn from a stale block) runs one group at a time.
-  __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
-                                             uint32_t attempt, uint64_t& result, bool& done, bool& oom) {
+  // `subgroup` splits a class's lanes further (lanes with equal class and
+  // subgroup form one group); malloc() passes 0, so a group is every lane of
+  // the class. `allocs` is the number of allocations a lane's `count` slices
+  // stand for: 1 for malloc, the run length for a bulk claim.
+  __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint32_t allocs,
+                                             uint64_t tid, uint32_t hint, uint32_t subgroup, uint32_t attempt,
+                                             uint64_t& result, bool& done, bool& oom) {
     const uint32_t me = wave::lane_id();
-    const uint64_t group = wave::match_groups(looking, tree);
+    const uint64_t group = wave::match_groups(looking, (uint64_t)tree | ((uint64_t)subgroup << 32));
     const int leader = wave::first_lane(group);
     const int last = wave::last_lane(group);
     const bool is_leader = (int)me == leader;
@@ -377,11 +393,11 @@ struct Gallatin {
     }
     // each group's last lane holds the inclusive total and does the one add
     // (reference block.cuh:245-264)
-    uint32_t merged = 0;
-    if ((int)me == last && has_block) merged = atom::add(&table.blocks[block].malloc_counter, total);
-    merged = wave::shfl(merged, last);
+    uint64_t merged = 0;
+    if ((int)me == last && has_block) merged = atom::add(&table.blocks[block].malloc_counter, (uint64_t)total);
+    merged = wave::shfl64(merged, last);
 
-    const uint32_t c0 = merged & kBlockCountMask;
+    const uint32_t c0 = counter_count(merged);
     const uint64_t allocation = (uint64_t)c0 + prefix;
     const bool should_replace = has_block && allocation <= 4095 && allocation + count > 4095;
     const bool valid = has_block && allocation + count <= kSlicesPerBlock;
@@ -392,7 +408,16 @@ struct Gallatin {
     // (reference block.cuh:267-281): in-range slices minus successful lanes.
     const uint32_t in_range =
         c0 >= kSlicesPerBlock ? 0u : ((c0 + total < kSlicesPerBlock ? c0 + total : kSlicesPerBlock) - c0);
-    const uint32_t written_off = has_block ? in_range - wave::popc(valid_lanes) : 0u;
+    // allocations the successful lanes hold: one per lane, or a lane's run
+    // length for the batched bulk claims
+    uint32_t held;
+    if ((wave::ballot(allocs != 1u) & looking) == 0) {
+      held = wave::popc(valid_lanes);
+    } else {
+      uint32_t unused;
+      wave::group_scan(group, valid ? allocs : 0u, 7, unused, held);
+    }
+    const uint32_t written_off = has_block ? in_range - held : 0u;
     bool completed = false;
     if (is_leader && written_off > 0) {
       const uint32_t old = atom::add(&table.blocks[block].free_counter, written_off);
@@ -400,8 +425,8 @@ struct Gallatin {
     }
     // a stale pinned read: the block now serves another class; what the valid
     // lanes drew is given back (reference gallatin.cuh:1037-1056)
-    const bool stale = has_block && (merged >> kBlockTreeShift) != tree;
-    const uint32_t give_back = stale ? wave::popc(valid_lanes) : 0u;
+    const bool stale = has_block && counter_class(merged) != tree;
+    const uint32_t give_back = stale ? held : 0u;
     // The group whose range crosses slice 4095 replaces the block (reference
     // :992, :1015-1026). A group that finds the block already spent, or
     // serving another class, replaces it too: a claim made through a stale
@@ -443,15 +468,16 @@ struct Gallatin {
   // structurizer runs a loop's exit block once, after the loop, for all lanes
   // that left, so per-group work must not sit in one. `oom` is set when the
   // failure is a confirmed out-of-memory, which a further call would repeat.
-  __device__ __forceinline__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint64_t tid, uint32_t hint,
-                                              bool& oom) {
+  __device__ __forceinline__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint32_t allocs,
+                                                              uint64_t tid, uint32_t hint, uint32_t subgroup,
+                                                              bool& oom) {
     uint64_t result = utils::kFail;
     bool done = false;
     oom = false;
     uint64_t looking = wave::active_mask();
     for (int attempt = 0; attempt < kSliceAttempts && looking; attempt++) {
       if ((looking >> wave::lane_id()) & 1ull)
-        claim_pass(looking, tree, count, tid, hint, (uint32_t)attempt, result, done, oom);
+        claim_pass(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)attempt, result, done, oom);
       looking &= ~wave::ballot(done);
     }
     return result;
@@ -463,9 +489,9 @@ struct Gallatin {
     int64_t block = request_new_block_from_tree(tree, tid, last);
     if (block < 0) return utils::kFail;
     Block* b = &table.blocks[block];
-    uint32_t old_merged = atom::add(&b->malloc_counter, kSlicesPerBlock);
-    uint32_t old_count = old_merged & kBlockCountMask;
-    if ((old_merged >> kBlockTreeShift) != tree) count_error(kErrFillClass);
+    const uint64_t old_merged = atom::add(&b->malloc_counter, (uint64_t)kSlicesPerBlock);
+    const uint32_t old_count = counter_count(old_merged);
+    if (counter_class(old_merged) != tree) count_error(kErrFillClass);
     atom::add(&b->free_counter, 4095u - old_count);
     if (last) gather_after_last(tree);
     if (old_count != 0) {
@@ -524,8 +550,10 @@ struct Gallatin {
 
   // --------------------------------------------------------------- malloc
   // malloc with the slot selector and hash id given explicitly; malloc()
-  // passes the hardware CU id and the global thread id.
-  __device__ __forceinline__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint) {
+  // passes the hardware CU id and the global thread id. `subgroup` (0 for
+  // malloc) splits the lanes of one class into separate claim groups: the
+  // coalescing width of the batched kernels.
+  __device__ __forceinline__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint, uint32_t subgroup = 0) {
     uint32_t tree = get_tree_id_from_size(size);
     uint32_t count = 1;
     if (tree >= kNumTrees) {
@@ -537,7 +565,45 @@ struct Gallatin {
     uint64_t offset = utils::kFail;
     bool oom = false;
     for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail && !oom; attempt++)
-      offset = malloc_slice_allocation(tree, count, tid, hint, oom);
+      offset = malloc_slice_allocation(tree, count, 1u, tid, hint, subgroup, oom);
+    if (offset == utils::kFail) return nullptr;
+    return offset_to_allocation(offset, tree);
+  }
+
+  // Bytes between consecutive allocations of a run of `size`-byte requests
+  // (0: the size takes the block or segment path, which has no runs).
+  __host__ __device__ static uint64_t run_stride(uint64_t size) {
+    const uint32_t tree = size < smallest ? 0u : (uint32_t)(utils::get_first_bit_bigger(size) - kSmallestBits);
+    if (tree < kNumTrees) return smallest << tree;
+    if (utils::get_first_bit_bigger(size) - kSmallestTreeBits >= 0) return 0;
+    return (uint64_t)biggest << (tree - (kNumTrees - 1));
+  }
+
+  // Bulk claim for the batched kernels (no counterpart in the reference,
+  // whose callers malloc one request per lane): k requests of one size class
+  // served from consecutive slices of one block with the lane's share of one
+  // group add, exactly as a k-times larger multi-slice request is claimed
+  // (reference gallatin.cuh:865-1078), but accounted as k allocations, each
+  // freed on its own (or all at once with free_run). Returns the first
+  // allocation; allocation r is at + r * run_stride(size). nullptr if the
+  // size has no runs or k * slices-per-request exceeds a block.
+  __device__ __forceinline__ void* malloc_run_at(uint64_t size, uint32_t k, uint64_t tid, uint32_t hint,
+                                                 uint32_t subgroup) {
+    uint32_t tree = get_tree_id_from_size(size);
+    uint32_t per = 1;
+    bool ok = k >= 1 && k <= 64;
+    if (tree >= kNumTrees) {
+      if (utils::get_first_bit_bigger(size) - kSmallestTreeBits >= 0) ok = false;
+      per = ok ? 1u << (tree - (kNumTrees - 1)) : 1u;
+      tree = kNumTrees - 1;
+    }
+    if (per * k > kSlicesPerBlock) ok = false;
+    uint64_t offset = utils::kFail;
+    if (ok) {
+      bool oom = false;
+      for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail && !oom; attempt++)
+        offset = malloc_slice_allocation(tree, per * k, k, tid, hint, subgroup, oom);
+    }
     if (offset == utils::kFail) return nullptr;
     return offset_to_allocation(offset, tree);
   }
@@ -554,14 +620,21 @@ struct Gallatin {
   // out again), then one atomic instruction in which each group's first lane
   // adds the group's size to its block's free counter. A block whose count
   // completes is recycled, one at a time.
-  __device__ __forceinline__ void free_offset(uint64_t offset) {
+  __device__ __forceinline__ void free_offset(uint64_t offset, uint32_t allocs = 1u) {
     const uint64_t block = offset / kSlicesPerBlock;
-    const uint64_t group = wave::match_groups(wave::active_mask(), block);
+    const uint64_t active = wave::active_mask();
+    const uint64_t group = wave::match_groups(active, block);
     const bool is_leader = (int)wave::lane_id() == wave::first_lane(group);
+    uint32_t n;
+    if ((wave::ballot(allocs != 1u) & active) == 0) {
+      n = wave::popc(group);
+    } else {
+      uint32_t unused;
+      wave::group_scan(group, allocs, 7, unused, n);
+    }
     atom::fence_release();
     bool complete = false;
     if (is_leader) {
-      const uint32_t n = wave::popc(group);
       const uint32_t old = atom::add(&table.blocks[block].free_counter, n);
       complete = (old + n == kSlicesPerBlock);
     }
@@ -576,6 +649,21 @@ struct Gallatin {
     if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
     atom::fence_release();
     segment_tree.return_multiple(segment, tree - kNumTrees - 1);
+  }
+
+  // Free all k allocations of a run from malloc_run_at at once: one share
+  // of one group add, as k separate frees would add one each.
+  __device__ __forceinline__ void free_run(void* first, uint32_t k) {
+    const uint64_t byte = (uint64_t)((char*)first - table.memory);
+    const uint64_t segment = byte / bytes_per_segment;
+    const uint32_t tree = byte < table.num_segments * bytes_per_segment ? table.read_tree_id(segment) : kFreeChunk;
+    if (tree >= kNumTrees || k == 0 || k > 64) {
+      atom::add(&control[kCtlBadFrees], 1u);
+      return;
+    }
+    const uint64_t offset = (byte - segment * bytes_per_segment) / table_type::tree_alloc_size(tree) +
+                            segment * table_type::max_allocations_per_segment;
+    free_offset(offset, k);
   }
 
   // reference gallatin.cuh:1378-1461
@@ -653,7 +741,7 @@ static __global__ void gallatin_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 }
 static __global__ void gallatin_init_blocks(Block* b, uint64_t n) {
   for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    b[i] = Block{4097u, 0u};  // Block::init, reference block.cuh:64-68
+    b[i] = Block{4097ull, 0u, 0u};  // Block::init, reference block.cuh:64-68
 }
 static __global__ void gallatin_init_segments(segment_counters* s, uint64_t n) {
   for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)

@@ -19,9 +19,10 @@
 namespace gallatin {
 namespace allocators {
 
-constexpr uint32_t kBlockTreeShift = 20;  // class id lives above bit 20 of malloc_counter
-constexpr uint32_t kBlockCountMask = (1u << kBlockTreeShift) - 1u;
 constexpr uint32_t kSlicesPerBlock = 4096;
+// The snapshot image packs a block's malloc counter as (class << 20) | count,
+// the reference's own encoding (block.cuh:215-224).
+constexpr uint32_t kSnapshotClassShift = 20;
 constexpr uint32_t kFreeChunk = 0xFFFFu;
 
 // Each block's counters get a 128-byte line of their own. Returned atomics
@@ -29,11 +30,24 @@ constexpr uint32_t kFreeChunk = 0xFFFFu;
 // whichever words of the line they target), while distinct lines proceed in
 // parallel; the pinned blocks of a class are usually neighbours, so packed
 // 8-byte records would put a whole class behind one line.
+//
+// malloc_counter holds the class in its high 32 bits and the slices handed
+// out in its low 32 bits (the reference packs them as class << 20 | count in
+// 32 bits). Every claim adds its group's slice total, also to a block that
+// is already spent; with multi-slice requests one wave can add 4096 at once,
+// so a few hundred late claims would carry a 20-bit count into the class
+// bits and wrap it back into the valid range.
 struct alignas(128) Block {
-  uint32_t malloc_counter;  // (class << 20) | slices handed out
+  uint64_t malloc_counter;  // (class << 32) | slices handed out
   uint32_t free_counter;    // slices returned (or written off) so far
+  uint32_t unused;
 };
 static_assert(sizeof(Block) == 128, "one block per cache line");
+__host__ __device__ constexpr uint64_t block_counter(uint32_t tree, uint32_t count) {
+  return ((uint64_t)tree << 32) | count;
+}
+__host__ __device__ constexpr uint32_t counter_class(uint64_t m) { return (uint32_t)(m >> 32); }
+__host__ __device__ constexpr uint32_t counter_count(uint64_t m) { return (uint32_t)m; }
 
 struct segment_counters {
   int32_t active;        // block grants left (-1: none / segment not formatted)
@@ -167,8 +181,8 @@ struct alloc_table {
     }
     // Block::init_malloc. The old value is read so that the reset is
     // performed before the block can be published anywhere.
-    const uint32_t old_counter = atom::exch(&blocks[block].malloc_counter, tree << kBlockTreeShift);
-    if (old_counter == ~0u) count_error_at(control, kErrInitCounter);
+    const uint64_t old_counter = atom::exch(&blocks[block].malloc_counter, block_counter(tree, 0));
+    if (old_counter == ~0ull) count_error_at(control, kErrInitCounter);
     if (read_tree_id(segment) != tree) count_error_at(control, kErrGrantClass);
     if (active_count == 0) last = true;
     return block;

@@ -194,6 +194,41 @@ int gal_latency_samples(gal_allocator* h, uint32_t* ticks, uint64_t cap, uint64_
 /* Copy n bytes of the pool starting at byte offset off to host memory. */
 int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n);
 
+/* ---- batches of uneven requests ----
+ * A batch holds up to `capacity` request sizes. Serving it bins the requests
+ * by the class they claim from (a stable counting sort on the device), then
+ * one persistent launch serves the binned order, `run` consecutive requests
+ * per lane (1..16): a lane whose requests share a slice class claims them
+ * with one bulk claim (Gallatin::malloc_run_at: consecutive slices of one
+ * block, the lane's share of one group add) and frees them with one
+ * free_run. Lanes of one class form claim groups of `width` lanes (1..64, a
+ * power of two). No counterpart in the reference, whose callers malloc one
+ * request per thread; it serves the same requests from the same blocks and
+ * counters (Gallatin::malloc/free, gallatin.cuh:1186-1461). */
+typedef struct gal_batch gal_batch;
+#define GAL_FORM_PER_THREAD 0u /* one thread per request, request order: malloc, tag, check, free */
+#define GAL_FORM_BATCHED 1u    /* bin + persistent malloc, tag, check, free; launched directly */
+#define GAL_FORM_GRAPH 2u      /* the same launches captured once into a hipGraph and replayed */
+int gal_batch_create(gal_allocator* h, uint64_t capacity, gal_batch** out);
+int gal_batch_destroy(gal_batch* b);
+/* n sizes 16 << k, k uniform in [0, size_classes), drawn on the device from (seed, request) */
+int gal_batch_generate(gal_batch* b, uint64_t n, uint64_t seed, uint32_t size_classes);
+/* n sizes copied from host memory; seed feeds the payload tags */
+int gal_batch_set_sizes(gal_batch* b, const uint64_t* sizes, uint64_t n, uint64_t seed);
+/* serve the batch `replays` times in the given form (each request malloc'd,
+ * tagged, checked and freed); out->total_ms covers all replays. width and
+ * run are ignored by GAL_FORM_PER_THREAD. */
+int gal_batch_run(gal_batch* b, uint32_t form, uint32_t width, uint32_t run, uint32_t replays, uint32_t verify,
+                  gal_run_result* out);
+/* batched malloc of every request (results kept in the batch), and the
+ * batched free of those results, one by one, in the same binned order */
+int gal_batch_malloc(gal_batch* b, uint32_t width, uint32_t run, gal_run_result* out);
+int gal_batch_free(gal_batch* b, gal_run_result* out);
+/* results of gal_batch_malloc as byte offsets from the pool base (GAL_NULL_OFFSET: miss) */
+int gal_batch_offsets(gal_batch* b, uint64_t* offsets, uint64_t cap);
+/* request indices in the binned order of the last binning */
+int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap);
+
 /* Agent-scope 32-bit atomic add rate of this GPU, in returned adds per
  * second. GAL_ATOMIC_DISTINCT: `threads` lanes each add `iters` times to
  * their own word. GAL_ATOMIC_WAVE_HOT: the first lane of each wave of
