@@ -128,6 +128,21 @@ __global__ void bin_scatter_kernel(const uint64_t* sizes, uint64_t n, const uint
 // Lanes of one class are split into claim groups of 2^width_log2 lanes, each
 // with its own slot hint.
 constexpr uint32_t kMaxRun = 16;
+// A lane holds at most this many bytes of one run at once: the persistent
+// grid keeps thousands of waves on neighbouring spans of one bin, so runs of
+// large requests would otherwise hold far more memory than the pool has.
+constexpr uint64_t kRunBytes = 64 * 1024;
+// A claim that finds memory short is retried this many times, with growing
+// sleeps, before its requests count as misses: in a batch most of the
+// memory is held by requests of the same launch that are about to be freed.
+constexpr int kBatchRetries = 12;
+
+__device__ inline void batch_backoff(int t) {
+  if (t < 4)
+    __builtin_amdgcn_s_sleep(16);
+  else
+    __builtin_amdgcn_s_sleep(127);
+}
 
 template <class A>
 __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes, const uint32_t* perm, uint64_t n,
@@ -150,24 +165,39 @@ __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes,
     bool bulk = (mode & kBatchMalloc) && k > 1 && stride != 0;
     for (uint32_t r = 1; r < k && bulk; r++) bulk = A::run_stride(sizes[perm[first + r]]) == stride;
     if (bulk) {
-      char* base = (char*)a->malloc_run_at(size0, k, tid, hint, sub);
-      if (base) {
-        for (uint32_t r = 0; r < k; r++) {
-          uint64_t* p = (uint64_t*)(base + r * stride);
-          const uint32_t req = perm[first + r];
-          nb += !tag_in(p, gw::payload_tag(seed, req), verify);
-          if (mode & kBatchFree) nb += !tag_out(p, gw::payload_tag(seed, req), verify);
-          if (ptrs) ptrs[req] = (mode & kBatchFree) ? nullptr : p;
+      // sub-runs of at most kRunBytes
+      const uint32_t cap = stride >= kRunBytes ? 1u : (uint32_t)(kRunBytes / stride);
+#pragma unroll 1
+      for (uint32_t r0 = 0; r0 < k; r0 += cap) {
+        const uint32_t kk = k - r0 < cap ? k - r0 : cap;
+        char* base = nullptr;
+#pragma unroll 1
+        for (int t = 0; t <= kBatchRetries && !base; t++) {
+          if (t) batch_backoff(t);
+          base = kk > 1 ? (char*)a->malloc_run_at(size0, kk, tid, hint, sub)
+                        : (char*)a->malloc_at(size0, tid, hint, sub);
         }
-        nm += k;
-        if (mode & kBatchFree) {
-          a->free_run(base, k);
-          nf += k;
+        if (base) {
+          for (uint32_t r = 0; r < kk; r++) {
+            uint64_t* p = (uint64_t*)(base + r * stride);
+            const uint32_t req = perm[first + r0 + r];
+            nb += !tag_in(p, gw::payload_tag(seed, req), verify);
+            if (mode & kBatchFree) nb += !tag_out(p, gw::payload_tag(seed, req), verify);
+            if (ptrs) ptrs[req] = (mode & kBatchFree) ? nullptr : p;
+          }
+          nm += kk;
+          if (mode & kBatchFree) {
+            if (kk > 1)
+              a->free_run(base, kk);
+            else
+              a->free(base);
+            nf += kk;
+          }
+        } else {
+          nx += kk;
+          if (ptrs)
+            for (uint32_t r = 0; r < kk; r++) ptrs[perm[first + r0 + r]] = nullptr;
         }
-      } else {
-        nx += k;
-        if (ptrs)
-          for (uint32_t r = 0; r < k; r++) ptrs[perm[first + r]] = nullptr;
       }
     } else {
       // one request at a time
@@ -177,7 +207,12 @@ __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes,
         const uint64_t tag = gw::payload_tag(seed, req);
         uint64_t* p;
         if (mode & kBatchMalloc) {
-          p = (uint64_t*)a->malloc_at(sizes[req], tid, hint, sub);
+          p = nullptr;
+#pragma unroll 1
+          for (int t = 0; t <= kBatchRetries && !p; t++) {
+            if (t) batch_backoff(t);
+            p = (uint64_t*)a->malloc_at(sizes[req], tid, hint, sub);
+          }
           if (p) {
             nm++;
             nb += !tag_in(p, tag, verify);

@@ -36,6 +36,7 @@ namespace allocators {
 // lane (or no concurrent frees) a retry repeats an identical failure.
 constexpr int kMallocLoopAttempts = 10;     // GALLATIN_MALLOC_LOOP_ATTEMPTS
 constexpr int kSliceAttempts = 500;         // GALLATIN_MAX_ATTEMPTS
+constexpr int kBusyPasses = 1 << 16;        // safety bound on passes that wait for a busy slot
 constexpr int kRequestBlockAttempts = 1000; // REQUEST_BLOCK_MAX_ATTEMPTS
 constexpr int kRequestBlockPasses = 1 << 20; // safety bound on lost-race retries
 constexpr int kRequestBlockOomAttempts = 2; // failures with no segment anywhere
@@ -341,7 +342,7 @@ struct Gallatin {
   // stand for: 1 for malloc, the run length for a bulk claim.
   __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint32_t allocs,
                                              uint64_t tid, uint32_t hint, uint32_t subgroup, uint32_t attempt,
-                                             uint64_t& result, bool& done, bool& oom) {
+                                             uint64_t& result, bool& done, bool& oom, bool& waited) {
     const uint32_t me = wave::lane_id();
     const uint64_t group = wave::match_groups(looking, (uint64_t)tree | ((uint64_t)subgroup << 32));
     const int leader = wave::first_lane(group);
@@ -452,6 +453,7 @@ struct Gallatin {
       }
     }
     if (wave::ballot(busy) & looking) wave::backoff(8);
+    waited = busy;
     if (empty && wave::shfl((uint32_t)refilled, leader) == 0u) {  // class out of memory
       done = true;
       oom = true;
@@ -475,10 +477,16 @@ struct Gallatin {
     bool done = false;
     oom = false;
     uint64_t looking = wave::active_mask();
-    for (int attempt = 0; attempt < kSliceAttempts && looking; attempt++) {
+    // A pass spent waiting for a busy slot is not a failed attempt: the
+    // reference's attempt budget counts claims that failed, and a single
+    // lane never waits. Waiting is bounded separately.
+    uint32_t tries = 0;
+    for (int pass = 0; pass < kBusyPasses && looking; pass++) {
+      bool waited = false;
       if ((looking >> wave::lane_id()) & 1ull)
-        claim_pass(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)attempt, result, done, oom);
-      looking &= ~wave::ballot(done);
+        claim_pass(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)pass, result, done, oom, waited);
+      if (!done && !waited) tries++;
+      looking &= ~wave::ballot(done || tries >= (uint32_t)kSliceAttempts);
     }
     return result;
   }

@@ -45,7 +45,12 @@ struct veb_tree {
 
   __device__ bool query(uint64_t v) const { return (load_word(num_layers - 1, v >> 6) >> (v & 63)) & 1ull; }
 
-  // clear item v; clear summary bits upward while a word empties
+  // Clear item v; clear summary bits upward while a word empties. After
+  // clearing a summary bit the word below is read again: an insert into it
+  // between the emptying and the clear saw an empty word and set the very
+  // bit being cleared, and without a restore its item would stay set but
+  // unreachable from the summaries (a free segment no search can find). With
+  // one lane the word is still empty and nothing is restored.
   __device__ bool remove(uint64_t v) {
     uint64_t high = v >> 6;
     int low = (int)(v & 63);
@@ -53,10 +58,16 @@ struct veb_tree {
     uint64_t old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
     if (!(old & (1ull << low))) return false;
     while (utils::popc64(old) == 1 && layer > 0) {
+      const uint64_t child = high;
+      const int child_layer = layer;
       layer--;
       low = (int)(high & 63);
       high >>= 6;
       old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
+      if (atom::load(&bits[child_layer][child]) != 0) {
+        atom::fetch_or(&bits[layer][high], 1ull << low);
+        break;
+      }
     }
     return true;
   }

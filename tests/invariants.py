@@ -3,6 +3,7 @@
 Live-set checks (any time): no two live allocations overlap, each starts on
 its granted size's alignment, each lies inside the pool.
 Quiescent checks (no allocation live, no call in flight) on a snapshot:
+  - every item of a class tree is reachable from its summaries
   - a segment is free (chunk id 0xFFFF) exactly when its segment-tree bit is set
   - a class tree only lists segments formatted for that class
   - per formatted segment: active == nb - 1 - queue_alloc + queue_free,
@@ -82,8 +83,16 @@ def check_quiescent(image, seg_bytes, min_size):
     assert np.array_equal(free_bits, chunk == 0xFFFF), "segment tree and chunk ids disagree"
 
     for c in range(T):
-        listed = bits_of(s["sub_trees"][c][-1])
+        layers = s["sub_trees"][c]
+        listed = bits_of(layers[-1])
         assert all(chunk[x] == c for x in listed), f"class {c} tree lists a segment of another class"
+        # every listed segment is reachable: a word that holds items has its
+        # summary bit set (class trees only change by single inserts/removes)
+        for upper, lower in zip(layers[:-1], layers[1:]):
+            summary = set(bits_of(upper))
+            holding = [w for w, val in enumerate(np.asarray(lower, dtype=np.uint64).tolist()) if val]
+            lost = [w for w in holding if w not in summary]
+            assert not lost, f"class {c} tree: words {lost[:8]} hold items but their summary bits are clear"
 
     slots = s["slots"].astype(np.int64)
     pinned = slots[slots > 0] - 1
