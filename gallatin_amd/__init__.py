@@ -218,6 +218,12 @@ class Gallatin:
         )
         return buf[: n.value].astype(np.float64) * tick.value
 
+    def free_bytes(self):
+        """Bytes in free segments (the allocator's gauge, gal_free_bytes)."""
+        out = ctypes.c_uint64()
+        _capi.check("gal_free_bytes", _lib.gal_free_bytes(self._h, ctypes.byref(out)))
+        return out.value
+
     def read_pool(self, offset, nbytes):
         buf = np.zeros(nbytes, dtype=np.uint8)
         _capi.check(
@@ -312,6 +318,70 @@ class Batch:
         p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         _capi.check("gal_batch_order", _lib.gal_batch_order(self._b, p, self.n))
         return out[: self.n]
+
+
+# ---- serving loop (no reference counterpart; every page is a Gallatin malloc/free)
+REQUEST_DTYPE = np.dtype([("arrival_ns", "<u8"), ("id", "<u8"), ("pages", "<u4"), ("page_bytes", "<u4")])
+SERVE_DEFAULT_HEADROOM = _capi.GAL_SERVE_DEFAULT_HEADROOM
+
+
+def generate_requests(n, rate_per_s, median_pages=1024.0, sigma=1.0, max_pages=32768, page_bytes=16384,
+                      size_classes=9, seed=0):
+    """Sequence requests (gal_serve_generate): Poisson arrivals at rate_per_s
+    (0: all at time 0), lognormal lengths in pages, fixed page_bytes or, with
+    page_bytes=0, 16 << k bytes with k uniform in [0, size_classes)."""
+    w = _capi.GalServeWorkload(n, float(rate_per_s), float(median_pages), float(sigma), max_pages, page_bytes,
+                               size_classes, 0, seed)
+    out = np.zeros(n, dtype=REQUEST_DTYPE)
+    _capi.check("gal_serve_generate", _lib.gal_serve_generate(ctypes.byref(w), out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+class Serving:
+    """A device-resident serving loop over one allocator (gal_serve_* in
+    include/gallatin_amd.h): sequences arrive, are admitted by free pool
+    bytes, grow one page per step and free their pages when done."""
+
+    def __init__(self, allocator, capacity):
+        h = ctypes.c_void_p()
+        _capi.check("gal_serve_create", _lib.gal_serve_create(allocator._h, capacity, ctypes.byref(h)))
+        self._s = h
+        self.allocator = allocator
+        self.n = 0
+
+    def close(self):
+        if self._s:
+            _capi.check("gal_serve_destroy", _lib.gal_serve_destroy(self._s))
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def load(self, requests):
+        arr = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
+        _capi.check("gal_serve_load", _lib.gal_serve_load(self._s, arr.ctypes.data_as(ctypes.c_void_p), len(arr)))
+        self.n = len(arr)
+
+    def run(self, slots=16384, headroom=SERVE_DEFAULT_HEADROOM, deadline_s=60.0):
+        r = _capi.GalServeResult()
+        _capi.check("gal_serve_run", _lib.gal_serve_run(self._s, slots, headroom, deadline_s, ctypes.byref(r)))
+        return r.as_dict()
+
+    def results(self):
+        """(checksums, admit_ns, done_ns) per request of the last run; an
+        aborted request has checksum 0 and times 2**64-1."""
+        n = max(1, self.n)
+        out = [np.zeros(n, dtype=np.uint64) for _ in range(3)]
+        p = ctypes.POINTER(ctypes.c_uint64)
+        _capi.check(
+            "gal_serve_results",
+            _lib.gal_serve_results(self._s, out[0].ctypes.data_as(p), out[1].ctypes.data_as(p),
+                                   out[2].ctypes.data_as(p), self.n),
+        )
+        return tuple(a[: self.n] for a in out)
 
 
 # ---- device atomic rates (the bound of the slice path; no reference counterpart)

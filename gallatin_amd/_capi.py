@@ -79,6 +79,54 @@ class GalRunResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GalRequest(ctypes.Structure):
+    _fields_ = [("arrival_ns", u64), ("id", u64), ("pages", u32), ("page_bytes", u32)]
+
+
+class GalServeWorkload(ctypes.Structure):
+    _fields_ = [
+        ("requests", u64),
+        ("rate_per_s", ctypes.c_double),
+        ("median_pages", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("max_pages", u32),
+        ("page_bytes", u32),
+        ("size_classes", u32),
+        ("unused", u32),
+        ("seed", u64),
+    ]
+
+
+class GalServeResult(ctypes.Structure):
+    _fields_ = [
+        ("requests_done", u64),
+        ("requests_aborted", u64),
+        ("page_mallocs", u64),
+        ("page_frees", u64),
+        ("page_retries", u64),
+        ("admission_refusals", u64),
+        ("peak_active", u64),
+        ("peak_committed", u64),
+        ("alloc_errors", u64),
+        ("slots", u64),
+        ("headroom", u64),
+        ("kernel_ms", ctypes.c_double),
+        ("makespan_ms", ctypes.c_double),
+        ("page_malloc_p50_ns", ctypes.c_double),
+        ("page_malloc_p99_ns", ctypes.c_double),
+        ("page_malloc_max_ns", ctypes.c_double),
+        ("queue_p50_ns", ctypes.c_double),
+        ("queue_p99_ns", ctypes.c_double),
+        ("latency_p50_ns", ctypes.c_double),
+        ("latency_p99_ns", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+GAL_SERVE_DEFAULT_HEADROOM = (1 << 64) - 1
+
 _SIGNATURES = {
     "gal_supported": (ctypes.c_int, [ctypes.POINTER(u64), ctypes.c_int]),
     "gal_create": (ctypes.c_int, [u64, u64, u64, u64, u64, ctypes.POINTER(vp)]),
@@ -117,6 +165,16 @@ _SIGNATURES = {
     "gal_latency_samples": (
         ctypes.c_int,
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],
+    ),
+    "gal_free_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "gal_serve_generate": (ctypes.c_int, [ctypes.POINTER(GalServeWorkload), vp]),
+    "gal_serve_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
+    "gal_serve_destroy": (ctypes.c_int, [vp]),
+    "gal_serve_load": (ctypes.c_int, [vp, vp, u64]),
+    "gal_serve_run": (ctypes.c_int, [vp, u32, u64, ctypes.c_double, ctypes.POINTER(GalServeResult)]),
+    "gal_serve_results": (
+        ctypes.c_int,
+        [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), u64],
     ),
 }
 

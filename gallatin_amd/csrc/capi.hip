@@ -5,6 +5,8 @@
 // pointers. All device work runs on the handle's own stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -68,6 +70,23 @@ struct gal_batch {
   uint64_t** d_ptrs = nullptr;    // results of gal_batch_malloc
 };
 
+// a loaded serving workload and the device buffers of its runs
+struct gal_serve {
+  gal_allocator* h = nullptr;
+  uint64_t capacity = 0;
+  uint64_t n = 0;
+  void* d_reqs = nullptr;  // serve_req[capacity]
+  uint64_t* d_checksums = nullptr;
+  uint64_t* d_admit = nullptr;  // wall-clock ticks after the start, ~0: aborted
+  uint64_t* d_done = nullptr;
+  void* d_state = nullptr;                // serve_state
+  unsigned long long* d_hist = nullptr;   // per-page malloc latency histogram
+  std::vector<uint64_t> arrival_ns;       // host copy of the arrivals
+  std::vector<uint32_t> page_sizes;       // distinct page sizes of the loaded requests
+  uint64_t max_need = 0;                  // largest request, bytes
+  bool ran = false;
+};
+
 namespace {
 
 struct impl_ops {
@@ -87,6 +106,10 @@ struct impl_ops {
   void (*batch_bin)(gal_batch* b);
   void (*batch_pass)(gal_batch* b, uint32_t mode, uint32_t width_log2, uint32_t run, uint32_t verify);
   void (*batch_per_thread)(gal_batch* b, uint32_t verify);
+  // serving loop (serve.inl)
+  uint64_t (*serve_headroom)(gal_allocator* h, const std::vector<uint32_t>& page_sizes);
+  unsigned (*serve_resident_blocks)();
+  void (*serve_launch)(gal_serve* s, unsigned blocks, uint64_t headroom, uint64_t deadline_ticks);
 };
 
 // ---------------------------------------------------------------- kernels
@@ -291,6 +314,7 @@ __global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) 
 inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
 #include "batch.inl"
+#include "serve.inl"
 
 // ------------------------------------------------------ per-type impl
 template <uint64_t S, uint64_t M, uint64_t B>
@@ -434,10 +458,65 @@ struct impl {
                                                                            verify, b->h->d_counters);
   }
 
+  // Memory the admission rule keeps back for the classes the page sizes
+  // claim from: every slot of such a class holding a partly filled block,
+  // two segments per class attached and not yet used up, and, when a class
+  // has several blocks per segment, a tenth of the pool for recycled blocks
+  // of segments not listed again yet (a segment is listed again once a
+  // tenth of its blocks are back). Pages of the segment path need none.
+  static uint64_t serve_headroom(gal_allocator* h, const std::vector<uint32_t>& page_sizes) {
+    bool used[A::kNumTrees] = {};
+    bool slots_used[A::kNumTrees] = {};
+    for (uint32_t size : page_sizes) {
+      const uint32_t tree = size < M ? 0u : (uint32_t)(gu::get_first_bit_bigger(size) - A::kSmallestBits);
+      if (tree < A::kNumTrees) {
+        used[tree] = slots_used[tree] = true;
+        continue;
+      }
+      const int block_tree = gu::get_first_bit_bigger(size) - A::kSmallestTreeBits;
+      if (block_tree < 0) {
+        used[A::kNumTrees - 1] = slots_used[A::kNumTrees - 1] = true;  // big slices
+      } else if ((uint32_t)block_tree < A::kNumTrees) {
+        used[block_tree] = true;  // whole blocks
+      }
+    }
+    uint64_t bytes = 0;
+    bool hidden = false;
+    for (uint32_t c = 0; c < A::kNumTrees; c++) {
+      if (!used[c]) continue;
+      if (slots_used[c])
+        bytes += (uint64_t)ga::slot_region_of(c) * A::table_type::tree_alloc_size(c) * ga::kSlicesPerBlock;
+      bytes += 2 * S;
+      hidden |= A::table_type::blocks_in_segment(c) > 1;
+    }
+    if (hidden) bytes += h->info.num_segments * S / 10;
+    return bytes;
+  }
+  static unsigned serve_resident_blocks() {
+    static int resident = 0;
+    if (resident == 0) {
+      int per_cu = 0, cus = 0, device = 0;
+      if (hipGetDevice(&device) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel<A>, 256, 0) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1)
+        per_cu = 1, cus = 256;
+      resident = per_cu * cus;
+    }
+    return (unsigned)resident;
+  }
+  static void serve_launch(gal_serve* s, unsigned blocks, uint64_t headroom, uint64_t deadline_ticks) {
+    serve_state* st = (serve_state*)s->d_state;
+    serve_start_kernel<<<1, 64, 0, s->h->stream>>>(st);
+    serve_kernel<A><<<blocks, 256, 0, s->h->stream>>>((A*)s->h->dev, (const serve_req*)s->d_reqs, s->n, st,
+                                                      s->d_checksums, s->d_admit, s->d_done, s->d_hist, headroom,
+                                                      deadline_ticks);
+  }
+
   static constexpr impl_ops ops() {
     return impl_ops{S,         M,          B,       &create,       &destroy,           &describe,
                     &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
-                    &batch_bin, &batch_pass, &batch_per_thread};
+                    &batch_bin, &batch_pass, &batch_per_thread, &serve_headroom, &serve_resident_blocks,
+                    &serve_launch};
   }
 };
 
@@ -965,6 +1044,224 @@ int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap) {
   if (!b || !order || cap < b->n) return GAL_ERR_INVALID;
   if (hipStreamSynchronize(b->h->stream) != hipSuccess) return GAL_ERR_HIP;
   return hipMemcpy(order, b->d_perm, b->n * 4, hipMemcpyDeviceToHost) == hipSuccess ? GAL_OK : GAL_ERR_HIP;
+}
+
+int gal_free_bytes(gal_allocator* h, uint64_t* bytes) {
+  if (!h || !bytes) return GAL_ERR_INVALID;
+  uint32_t segs = 0;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  if (hipMemcpy(&segs, h->control + ga::kCtlFreeSegs, 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  *bytes = (uint64_t)segs * h->info.segment_size;
+  return GAL_OK;
+}
+
+// ---- serving loop
+static double unit_draw(uint64_t seed, uint64_t i, uint64_t salt) {
+  const uint64_t x = gu::splitmix64(seed ^ gu::splitmix64((i << 4) | salt));
+  return ((double)(x >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+}
+
+int gal_serve_generate(const gal_serve_workload* w, gal_request* out) {
+  if (!w || !out || w->requests == 0 || w->max_pages == 0 || !(w->median_pages >= 1.0) || !(w->sigma >= 0.0) ||
+      !(w->rate_per_s >= 0.0))
+    return GAL_ERR_INVALID;
+  if (w->page_bytes == 0 && (w->size_classes == 0 || w->size_classes > 28)) return GAL_ERR_INVALID;
+  if (w->page_bytes != 0 && w->page_bytes < 16) return GAL_ERR_INVALID;
+  const double two_pi = 6.283185307179586476925;
+  double t = 0.0;
+  for (uint64_t i = 0; i < w->requests; i++) {
+    if (w->rate_per_s > 0.0) t += -std::log(unit_draw(w->seed, i, 0)) / w->rate_per_s;
+    // Box-Muller: z standard normal
+    const double z = std::sqrt(-2.0 * std::log(unit_draw(w->seed, i, 1))) * std::cos(two_pi * unit_draw(w->seed, i, 2));
+    double len = std::ceil(w->median_pages * std::exp(w->sigma * z));
+    if (!(len >= 1.0)) len = 1.0;
+    if (len > (double)w->max_pages) len = (double)w->max_pages;
+    gal_request& r = out[i];
+    r.arrival_ns = (uint64_t)(t * 1e9);
+    r.id = i;
+    r.pages = (uint32_t)len;
+    r.page_bytes = w->page_bytes ? w->page_bytes
+                                 : (uint32_t)(16u << (gu::splitmix64(w->seed ^ gu::splitmix64((i << 4) | 3)) %
+                                                      w->size_classes));
+  }
+  return GAL_OK;
+}
+
+int gal_serve_destroy(gal_serve* s) {
+  if (!s) return GAL_ERR_INVALID;
+  if (s->h) (void)hipStreamSynchronize(s->h->stream);
+  void* bufs[] = {s->d_reqs, s->d_checksums, s->d_admit, s->d_done, s->d_state, s->d_hist};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  delete s;
+  return GAL_OK;
+}
+
+int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out) {
+  if (!h || !out || capacity == 0) return GAL_ERR_INVALID;
+  *out = nullptr;
+  gal_serve* s = new gal_serve();
+  s->h = h;
+  s->capacity = capacity;
+  const bool ok = hipMalloc(&s->d_reqs, capacity * sizeof(serve_req)) == hipSuccess &&
+                  hipMalloc(&s->d_checksums, capacity * 8) == hipSuccess &&
+                  hipMalloc(&s->d_admit, capacity * 8) == hipSuccess &&
+                  hipMalloc(&s->d_done, capacity * 8) == hipSuccess &&
+                  hipMalloc(&s->d_state, sizeof(serve_state)) == hipSuccess &&
+                  hipMalloc(&s->d_hist, kHistBuckets * sizeof(unsigned long long)) == hipSuccess;
+  if (!ok) {
+    gal_serve_destroy(s);
+    return GAL_ERR_OOM;
+  }
+  *out = s;
+  return GAL_OK;
+}
+
+int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n) {
+  if (!s || !reqs || n == 0 || n > s->capacity) return GAL_ERR_INVALID;
+  std::vector<serve_req> dev(n);
+  s->arrival_ns.resize(n);
+  s->page_sizes.clear();
+  s->max_need = 0;
+  const double tick_ns = s->h->tick_ns;
+  for (uint64_t i = 0; i < n; i++) {
+    if (reqs[i].pages == 0 || reqs[i].page_bytes < 16) return GAL_ERR_INVALID;
+    dev[i].arrival = (uint64_t)((double)reqs[i].arrival_ns / tick_ns);
+    dev[i].id = reqs[i].id;
+    dev[i].pages = reqs[i].pages;
+    dev[i].page_bytes = reqs[i].page_bytes;
+    s->arrival_ns[i] = reqs[i].arrival_ns;
+    if (std::find(s->page_sizes.begin(), s->page_sizes.end(), reqs[i].page_bytes) == s->page_sizes.end())
+      s->page_sizes.push_back(reqs[i].page_bytes);
+    const uint64_t need = (uint64_t)reqs[i].pages * reqs[i].page_bytes;
+    if (need > s->max_need) s->max_need = need;
+  }
+  if (hipMemcpy(s->d_reqs, dev.data(), n * sizeof(serve_req), hipMemcpyHostToDevice) != hipSuccess)
+    return GAL_ERR_HIP;
+  s->n = n;
+  s->ran = false;
+  return GAL_OK;
+}
+
+static double nearest_rank(std::vector<double>& v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  size_t k = (size_t)std::ceil(q * (double)v.size());
+  return v[k == 0 ? 0 : k - 1];
+}
+
+// q-quantile of a latency histogram, linear inside the bucket
+static double hist_quantile(const std::vector<unsigned long long>& hist, double q, double tick_ns) {
+  unsigned long long total = 0;
+  for (auto c : hist) total += c;
+  if (total == 0) return 0.0;
+  const double target = q * (double)total;
+  double cum = 0.0;
+  for (uint32_t b = 0; b < kHistBuckets; b++) {
+    if (hist[b] == 0) continue;
+    if (cum + (double)hist[b] >= target) {
+      double lo, width;
+      hist_range(b, lo, width);
+      return (lo + width * (target - cum) / (double)hist[b]) * tick_ns;
+    }
+    cum += (double)hist[b];
+  }
+  return 0.0;
+}
+
+int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadline_s, gal_serve_result* out) {
+  if (!s || s->n == 0 || slots == 0 || !(deadline_s > 0.0) || deadline_s > 3600.0) return GAL_ERR_INVALID;
+  gal_allocator* h = s->h;
+  if (headroom == GAL_SERVE_DEFAULT_HEADROOM) headroom = h->ops->serve_headroom(h, s->page_sizes);
+  uint64_t room = 0;
+  int rc = gal_free_bytes(h, &room);
+  if (rc != GAL_OK) return rc;
+  if (s->max_need + headroom > room) return GAL_ERR_INVALID;  // some request could never be admitted
+  const unsigned resident = h->ops->serve_resident_blocks();
+  unsigned blocks = (unsigned)((slots + 255u) / 256u);
+  if (blocks > resident) blocks = resident;
+  const uint64_t deadline_ticks = (uint64_t)(deadline_s * 1e9 / h->tick_ns);
+
+  gal_run_result base;
+  std::memset(&base, 0, sizeof(base));
+  rc = timed_run(h, &base, [&](launch_timer& t) {
+    if (hipMemsetAsync(s->d_state, 0, sizeof(serve_state), h->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_hist, 0, kHistBuckets * sizeof(unsigned long long), h->stream) != hipSuccess)
+      return GAL_ERR_HIP;
+    if (!t.begin()) return GAL_ERR_INVALID;
+    h->ops->serve_launch(s, blocks, headroom, deadline_ticks);
+    t.end(0);
+    return GAL_OK;
+  });
+  if (rc != GAL_OK) return rc;
+  s->ran = true;
+
+  serve_state st;
+  std::vector<unsigned long long> hist(kHistBuckets);
+  std::vector<uint64_t> admit(s->n), done(s->n);
+  if (hipMemcpy(&st, s->d_state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hist.data(), s->d_hist, kHistBuckets * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(admit.data(), s->d_admit, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(done.data(), s->d_done, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return GAL_ERR_HIP;
+  if (!out) return GAL_OK;
+  gal_serve_result& r = *out;
+  std::memset(&r, 0, sizeof(r));
+  r.requests_done = st.done;
+  r.requests_aborted = st.aborted;
+  r.page_mallocs = st.page_mallocs;
+  r.page_frees = st.page_frees;
+  r.page_retries = st.page_retries;
+  r.admission_refusals = st.refused;
+  r.peak_active = st.peak_active;
+  r.peak_committed = st.peak_committed;
+  r.alloc_errors = base.alloc_errors;
+  r.slots = (uint64_t)blocks * 256;
+  r.headroom = headroom;
+  r.kernel_ms = base.malloc_kernel_ms;
+  const double tick_ns = h->tick_ns;
+  std::vector<double> queue, latency;
+  queue.reserve(s->n);
+  latency.reserve(s->n);
+  uint64_t last = 0;
+  for (uint64_t i = 0; i < s->n; i++) {
+    if (done[i] == ~0ull || admit[i] == ~0ull) continue;
+    if (done[i] > last) last = done[i];
+    const double arr = (double)s->arrival_ns[i];
+    queue.push_back(std::max(0.0, (double)admit[i] * tick_ns - arr));
+    latency.push_back(std::max(0.0, (double)done[i] * tick_ns - arr));
+  }
+  r.makespan_ms = (double)last * tick_ns * 1e-6;
+  r.page_malloc_p50_ns = hist_quantile(hist, 0.50, tick_ns);
+  r.page_malloc_p99_ns = hist_quantile(hist, 0.99, tick_ns);
+  for (uint32_t b = kHistBuckets; b-- > 0;)
+    if (hist[b]) {
+      double lo, width;
+      hist_range(b, lo, width);
+      r.page_malloc_max_ns = (lo + width) * tick_ns;
+      break;
+    }
+  r.queue_p50_ns = nearest_rank(queue, 0.50);
+  r.queue_p99_ns = nearest_rank(queue, 0.99);
+  r.latency_p50_ns = nearest_rank(latency, 0.50);
+  r.latency_p99_ns = nearest_rank(latency, 0.99);
+  return GAL_OK;
+}
+
+int gal_serve_results(gal_serve* s, uint64_t* checksums, uint64_t* admit_ns, uint64_t* done_ns, uint64_t cap) {
+  if (!s || !s->ran || cap < s->n) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(s->h->stream) != hipSuccess) return GAL_ERR_HIP;
+  if (checksums && hipMemcpy(checksums, s->d_checksums, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return GAL_ERR_HIP;
+  const double tick_ns = s->h->tick_ns;
+  auto times = [&](uint64_t* dst, const uint64_t* src) {
+    if (!dst) return true;
+    if (hipMemcpy(dst, src, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    for (uint64_t i = 0; i < s->n; i++) dst[i] = dst[i] == ~0ull ? ~0ull : (uint64_t)((double)dst[i] * tick_ns);
+    return true;
+  };
+  if (!times(admit_ns, s->d_admit) || !times(done_ns, s->d_done)) return GAL_ERR_HIP;
+  return GAL_OK;
 }
 
 int gal_atomic_rate(uint32_t mode, uint64_t threads, uint32_t iters, uint32_t addresses, double* ops_per_s) {

@@ -78,6 +78,7 @@ struct Gallatin {
   static_assert(smallest <= biggest, "smallest <= biggest");
   static_assert(bytes_per_segment >= biggest * 4096, "a segment holds at least one block of the top class");
 
+  static constexpr uint64_t kBytesPerSegment = bytes_per_segment;
   static constexpr uint32_t kNumTrees = (uint32_t)(ilog2(biggest) - ilog2(smallest) + 1);
   static constexpr int kSmallestBits = ilog2(smallest);
   static constexpr int kSmallestTreeBits = ilog2(smallest * 4096);
@@ -125,6 +126,7 @@ struct Gallatin {
   __device__ __noinline__ int64_t gather_new_segment(uint32_t tree) {
     uint64_t segment = segment_tree.malloc_first();
     if (segment == utils::kFail) return -1;
+    atom::sub(&control[kCtlFreeSegs], 1u);
     if (!table.setup_segment(segment, tree)) {
       count_error(kErrSetup);
       return -1;
@@ -224,6 +226,7 @@ struct Gallatin {
       if (!table.reset_tree_id(segment, tree)) count_error(kErrChunkReset);
       atom::fence();
       segment_tree.insert_force_update(segment);
+      atom::add(&control[kCtlFreeSegs], 1u);
     }
   }
 
@@ -519,6 +522,7 @@ struct Gallatin {
     atom::fence_acquire();
     uint64_t first = segment_tree.gather_multiple(n);
     if (first != utils::kFail) {
+      atom::sub(&control[kCtlFreeSegs], (uint32_t)n);
       if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error(kErrSetup);
     }
     atom::fence_release();
@@ -657,6 +661,7 @@ struct Gallatin {
     if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
     atom::fence_release();
     segment_tree.return_multiple(segment, tree - kNumTrees - 1);
+    atom::add(&control[kCtlFreeSegs], tree - kNumTrees - 1);
   }
 
   // Free all k allocations of a run from malloc_run_at at once: one share
@@ -856,6 +861,9 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_b
     off += slot_region_of(c);
   }
   if (hipMemset(table_mem, 0, table_bytes) != hipSuccess) return fail();
+  if (nseg > 0xFFFFFFFFull) return fail();
+  const uint32_t free_segs = (uint32_t)nseg;
+  if (hipMemcpy(host.control + kCtlFreeSegs, &free_segs, 4, hipMemcpyHostToDevice) != hipSuccess) return fail();
   gallatin_fill_u32<<<detail::grid_for(nseg), 256>>>(host.table.chunk_ids, nseg, kFreeChunk);
   gallatin_init_blocks<<<detail::grid_for(nblocks), 256>>>(host.table.blocks, nblocks);
   gallatin_init_segments<<<detail::grid_for(nseg), 256>>>(host.table.segs, nseg);

@@ -78,6 +78,8 @@ enum control_word : uint32_t {
   kErrFreeClass = 12,     //   a free hit a block whose counter carries another class
   kErrOverReturn = 13,    //   a segment got back more blocks than it granted
   kErrSites = 14,
+  kCtlFreeSegs = 18,      // segments in the segment tree (free pool bytes / segment size); updated
+                          // right after each claim from / return to the tree
   kCtlStatBase = 20,      // activity counters of the rare paths:
   kStatReplace = 0,       //   spent pinned blocks swapped out
   kStatRefill = 1,        //   empty pinned slots refilled

@@ -32,5 +32,17 @@ __host__ __device__ inline uint64_t payload_tag(uint64_t seed, uint64_t index) {
   return mixed_key(seed, 0, index, 2) | 1ull;
 }
 
+// ---- serving: the header a sequence writes into each of its pages.
+// Word 0 links to the sequence's previous page; words 1 .. page_words-1
+// hold content derived from (request id, page index). A sequence folds every
+// content word it reads back into its checksum, so the checksum is a pure
+// function of (id, pages, page bytes) unless another sequence wrote into one
+// of its pages.
+__host__ __device__ inline uint32_t page_words(uint32_t page_bytes) { return page_bytes >= 64 ? 8u : page_bytes / 8u; }
+__host__ __device__ inline uint64_t page_word(uint64_t id, uint64_t page, uint64_t k) {
+  return utils::splitmix64(utils::splitmix64(id) ^ ((page << 3) | k));
+}
+__host__ __device__ inline uint64_t fold(uint64_t h, uint64_t x) { return utils::splitmix64(h ^ x); }
+
 }  // namespace workloads
 }  // namespace gallatin

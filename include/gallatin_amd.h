@@ -229,6 +229,92 @@ int gal_batch_offsets(gal_batch* b, uint64_t* offsets, uint64_t cap);
 /* request indices in the binned order of the last binning */
 int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap);
 
+/* Bytes in free segments (segments in the segment tree) right now: the
+ * allocator's own gauge, kept exact at every segment claim and return. No
+ * counterpart in the reference (its print_info counts the tree's bits,
+ * gallatin.cuh:1855-1897); the serving loop admits requests against it. */
+int gal_free_bytes(gal_allocator* h, uint64_t* bytes);
+
+/* ---- serving loop ----
+ * Sequences arrive over time and grow by one page per step, then free all
+ * their pages. One persistent launch serves every request (continuous
+ * batching): each lane of the grid is a sequence slot; an idle slot takes
+ * the next request, admits it once it has arrived and the pool has room for
+ * all of its pages (free segment bytes >= pages not yet allocated by
+ * admitted sequences + the new request's bytes + a headroom for partly used
+ * blocks), grows it page by page (malloc, write the page header, read the
+ * previous page back), then drains it page by page (read back, free) and
+ * reports a checksum of what it read. The checksum of a request depends
+ * only on (id, pages, page_bytes), alone or in any mix. No counterpart in the
+ * reference; every page is an ordinary Gallatin::malloc/free
+ * (gallatin.cuh:1186-1461) made from inside the loop. */
+typedef struct gal_request {
+  uint64_t arrival_ns; /* arrival after the start of the run */
+  uint64_t id;         /* seeds the page contents */
+  uint32_t pages;      /* pages the sequence grows to, >= 1 */
+  uint32_t page_bytes; /* bytes per page, >= 16 */
+} gal_request;
+
+typedef struct gal_serve_workload {
+  uint64_t requests;
+  double rate_per_s;      /* Poisson arrivals; 0: every request arrives at time 0 */
+  double median_pages;    /* lognormal length: median ... */
+  double sigma;           /* ... and sigma of log(length) */
+  uint32_t max_pages;     /* lengths are capped here */
+  uint32_t page_bytes;    /* 0: per-request 16 << k bytes, k uniform in [0, size_classes) */
+  uint32_t size_classes;
+  uint32_t unused;
+  uint64_t seed;
+} gal_serve_workload;
+
+/* Fills out[0 .. w->requests) with the workload's requests (host only, no
+ * GPU): request i has id i, a length min(max_pages, ceil(median *
+ * exp(sigma * z))) with z standard normal, and arrives an exponential gap of
+ * mean 1/rate after request i-1. Every draw is a function of (seed, i). */
+int gal_serve_generate(const gal_serve_workload* w, gal_request* out);
+
+typedef struct gal_serve gal_serve;
+typedef struct gal_serve_result {
+  uint64_t requests_done;      /* requests that finished */
+  uint64_t requests_aborted;   /* requests cut by the deadline (checksum 0) */
+  uint64_t page_mallocs;       /* pages allocated */
+  uint64_t page_frees;         /* pages freed */
+  uint64_t page_retries;       /* page mallocs that returned nullptr (retried next step) */
+  uint64_t admission_refusals; /* admission attempts refused for lack of room */
+  uint64_t peak_active;        /* most sequences in flight at once */
+  uint64_t peak_committed;     /* most admitted-but-unallocated page bytes */
+  uint64_t alloc_errors;       /* growth of the allocator's internal error count */
+  uint64_t slots;              /* sequence slots launched */
+  uint64_t headroom;           /* headroom bytes used by admission */
+  double kernel_ms;            /* the serving launch (HIP events on the handle's stream) */
+  double makespan_ms;          /* start of the loop to the last completion (device clock) */
+  double page_malloc_p50_ns;   /* per-page malloc latency (wave time inside malloc, */
+  double page_malloc_p99_ns;   /*   one sample per page, 1/16-octave histogram) */
+  double page_malloc_max_ns;
+  double queue_p50_ns;         /* arrival -> admission */
+  double queue_p99_ns;
+  double latency_p50_ns;       /* arrival -> last page freed */
+  double latency_p99_ns;
+} gal_serve_result;
+
+#define GAL_SERVE_DEFAULT_HEADROOM (~(uint64_t)0)
+int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out);
+int gal_serve_destroy(gal_serve* s);
+int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n);
+/* Serve the loaded requests in one persistent launch of `slots` sequence
+ * slots (rounded up to 256, capped at what the GPU holds at once). headroom
+ * GAL_SERVE_DEFAULT_HEADROOM: for each class the page sizes claim from, its
+ * pinned and overflow slots' blocks and two segments; plus a tenth of the
+ * pool when such a class has several blocks per segment (recycled blocks of
+ * a segment stay hidden until a tenth of them are back, reference
+ * gallatin.cuh:1646). Requests not finished deadline_s
+ * seconds after the start are aborted (their pages are freed). Returns
+ * GAL_ERR_INVALID if a request could never be admitted. */
+int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadline_s, gal_serve_result* out);
+/* per request of the last run: checksum (0 if aborted), admission and
+ * completion times in ns after the start (~0 if aborted) */
+int gal_serve_results(gal_serve* s, uint64_t* checksums, uint64_t* admit_ns, uint64_t* done_ns, uint64_t cap);
+
 /* Agent-scope 32-bit atomic add rate of this GPU, in returned adds per
  * second. GAL_ATOMIC_DISTINCT: `threads` lanes each add `iters` times to
  * their own word. GAL_ATOMIC_WAVE_HOT: the first lane of each wave of
