@@ -109,6 +109,7 @@ class GalServeResult(ctypes.Structure):
         ("peak_committed", u64),
         ("alloc_errors", u64),
         ("slots", u64),
+        ("run", u64),
         ("headroom", u64),
         ("kernel_ms", ctypes.c_double),
         ("makespan_ms", ctypes.c_double),
@@ -171,7 +172,7 @@ _SIGNATURES = {
     "gal_serve_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
     "gal_serve_destroy": (ctypes.c_int, [vp]),
     "gal_serve_load": (ctypes.c_int, [vp, vp, u64]),
-    "gal_serve_run": (ctypes.c_int, [vp, u32, u64, ctypes.c_double, ctypes.POINTER(GalServeResult)]),
+    "gal_serve_run": (ctypes.c_int, [vp, u32, u32, u64, ctypes.c_double, ctypes.POINTER(GalServeResult)]),
     "gal_serve_results": (
         ctypes.c_int,
         [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), u64],

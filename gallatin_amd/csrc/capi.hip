@@ -109,7 +109,7 @@ struct impl_ops {
   // serving loop (serve.inl)
   uint64_t (*serve_headroom)(gal_allocator* h, const std::vector<uint32_t>& page_sizes);
   unsigned (*serve_resident_blocks)();
-  void (*serve_launch)(gal_serve* s, unsigned blocks, uint64_t headroom, uint64_t deadline_ticks);
+  void (*serve_launch)(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks);
 };
 
 // ---------------------------------------------------------------- kernels
@@ -504,12 +504,12 @@ struct impl {
     }
     return (unsigned)resident;
   }
-  static void serve_launch(gal_serve* s, unsigned blocks, uint64_t headroom, uint64_t deadline_ticks) {
+  static void serve_launch(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks) {
     serve_state* st = (serve_state*)s->d_state;
     serve_start_kernel<<<1, 64, 0, s->h->stream>>>(st);
     serve_kernel<A><<<blocks, 256, 0, s->h->stream>>>((A*)s->h->dev, (const serve_req*)s->d_reqs, s->n, st,
                                                       s->d_checksums, s->d_admit, s->d_done, s->d_hist, headroom,
-                                                      deadline_ticks);
+                                                      deadline_ticks, run);
   }
 
   static constexpr impl_ops ops() {
@@ -1169,8 +1169,10 @@ static double hist_quantile(const std::vector<unsigned long long>& hist, double 
   return 0.0;
 }
 
-int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadline_s, gal_serve_result* out) {
-  if (!s || s->n == 0 || slots == 0 || !(deadline_s > 0.0) || deadline_s > 3600.0) return GAL_ERR_INVALID;
+int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s,
+                  gal_serve_result* out) {
+  if (!s || s->n == 0 || slots == 0 || run == 0 || run > kServeMaxRun || !(deadline_s > 0.0) || deadline_s > 3600.0)
+    return GAL_ERR_INVALID;
   gal_allocator* h = s->h;
   if (headroom == GAL_SERVE_DEFAULT_HEADROOM) headroom = h->ops->serve_headroom(h, s->page_sizes);
   uint64_t room = 0;
@@ -1189,7 +1191,7 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadli
         hipMemsetAsync(s->d_hist, 0, kHistBuckets * sizeof(unsigned long long), h->stream) != hipSuccess)
       return GAL_ERR_HIP;
     if (!t.begin()) return GAL_ERR_INVALID;
-    h->ops->serve_launch(s, blocks, headroom, deadline_ticks);
+    h->ops->serve_launch(s, blocks, run, headroom, deadline_ticks);
     t.end(0);
     return GAL_OK;
   });
@@ -1213,10 +1215,10 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadli
   r.page_frees = st.page_frees;
   r.page_retries = st.page_retries;
   r.admission_refusals = st.refused;
-  r.peak_active = st.peak_active;
-  r.peak_committed = st.peak_committed;
+  r.peak_committed = st.peak_committed.v;
   r.alloc_errors = base.alloc_errors;
   r.slots = (uint64_t)blocks * 256;
+  r.run = run;
   r.headroom = headroom;
   r.kernel_ms = base.malloc_kernel_ms;
   const double tick_ns = h->tick_ns;
@@ -1232,6 +1234,27 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadli
     latency.push_back(std::max(0.0, (double)done[i] * tick_ns - arr));
   }
   r.makespan_ms = (double)last * tick_ns * 1e-6;
+  // most sequences in flight at once: at each admission, the requests
+  // admitted so far minus those finished strictly before it
+  {
+    std::vector<uint64_t> a_t, d_t;
+    a_t.reserve(s->n);
+    d_t.reserve(s->n);
+    for (uint64_t i = 0; i < s->n; i++) {
+      if (admit[i] == ~0ull) continue;
+      a_t.push_back(admit[i]);
+      d_t.push_back(done[i] == ~0ull ? ~0ull : done[i]);
+    }
+    std::sort(a_t.begin(), a_t.end());
+    std::sort(d_t.begin(), d_t.end());
+    uint64_t peak = 0;
+    size_t finished = 0;
+    for (size_t k = 0; k < a_t.size(); k++) {
+      while (finished < d_t.size() && d_t[finished] < a_t[k]) finished++;
+      peak = std::max<uint64_t>(peak, (uint64_t)(k + 1 - finished));
+    }
+    r.peak_active = peak;
+  }
   r.page_malloc_p50_ns = hist_quantile(hist, 0.50, tick_ns);
   r.page_malloc_p99_ns = hist_quantile(hist, 0.99, tick_ns);
   for (uint32_t b = kHistBuckets; b-- > 0;)

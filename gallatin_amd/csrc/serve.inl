@@ -31,6 +31,7 @@
 constexpr uint32_t kSeqIdle = 0, kSeqWaiting = 1, kSeqGrowing = 2, kSeqDraining = 3, kSeqExhausted = 4;
 constexpr uint32_t kHistBuckets = 512;  // 16 per octave of wall-clock ticks
 constexpr uint32_t kFlushEvery = 16;
+constexpr uint32_t kFreeBatch = 16;  // lanes holding a drained claim before the wave frees them
 
 struct serve_req {
   uint64_t arrival;  // wall-clock ticks after the start
@@ -39,12 +40,16 @@ struct serve_req {
   uint32_t page_bytes;
 };
 
+// The counters every wave updates while it runs get a 128-byte line each:
+// returned atomics to one line are performed one after another.
+struct alignas(128) serve_line {
+  unsigned long long v;
+};
 struct serve_state {
-  unsigned long long cursor;     // next ticket
-  unsigned long long committed;  // admitted page bytes not yet allocated
-  unsigned long long active;     // sequences admitted and not finished
-  unsigned long long peak_active;
-  unsigned long long peak_committed;
+  serve_line cursor;          // next ticket
+  serve_line committed;       // admitted page bytes not yet allocated
+  serve_line peak_committed;  // largest value `committed` reached at an admission
+  // written once per wave, at its end
   unsigned long long page_mallocs, page_frees, page_retries;
   unsigned long long done, aborted, refused;
   unsigned long long t0;  // start of the loop (wall-clock ticks)
@@ -76,32 +81,82 @@ __device__ inline uint64_t wave_sum64(uint64_t x) {
   return x;
 }
 
+// Page headers are read and written 16 bytes at a time (pages are aligned
+// to at least 16 bytes): word 0 is the link, words 1 .. w-1 the content, w
+// in {2, 4, 8}.
+typedef uint64_t page_pair __attribute__((ext_vector_type(2)));
+
+__device__ inline void write_page(uint64_t* p, uint64_t link, uint64_t id, uint32_t page, uint32_t w) {
+  page_pair* q = (page_pair*)p;
+  q[0] = page_pair{link, gw::page_word(id, page, 1)};
+  if (w > 2) q[1] = page_pair{gw::page_word(id, page, 2), gw::page_word(id, page, 3)};
+  if (w > 4) {
+    q[2] = page_pair{gw::page_word(id, page, 4), gw::page_word(id, page, 5)};
+    q[3] = page_pair{gw::page_word(id, page, 6), gw::page_word(id, page, 7)};
+  }
+}
+
+// fold words 1 .. w-1 of page p into h; returns the link word
+__device__ inline uint64_t read_page(const uint64_t* p, uint32_t w, uint64_t& h) {
+  const page_pair* q = (const page_pair*)p;
+  const page_pair a = q[0];
+  page_pair b = {0, 0}, c = {0, 0}, d = {0, 0};
+  if (w > 2) b = q[1];
+  if (w > 4) {
+    c = q[2];
+    d = q[3];
+  }
+  h = gw::fold(h, a.y);
+  if (w > 2) h = gw::fold(gw::fold(h, b.x), b.y);
+  if (w > 4) h = gw::fold(gw::fold(gw::fold(gw::fold(h, c.x), c.y), d.x), d.y);
+  return a.x;
+}
+
 __global__ void serve_start_kernel(serve_state* st) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atom::store(&st->t0, (unsigned long long)wall_ticks());
 }
+
+// Pages of one sequence are claimed `run` at a time (1..64): a growing lane
+// whose reserved pages are used up claims the next min(run, pages left)
+// pages with one bulk claim (Gallatin::malloc_run_at: consecutive slices of
+// one block, accounted as separate allocations), then takes one reserved
+// page per step. The first page of each claim carries a mark in its link
+// word; draining (newest page first) ends a claim's pages at that mark and
+// frees them with one free_run. run == 1 is one malloc and one free per
+// page. Pages of the block and segment paths are always claimed one by one.
+constexpr uint32_t kServeMaxRun = 64;
+constexpr uint64_t kRunStart = 1ull;  // link-word mark of a claim's first page (pages are 16-B aligned)
 
 template <class A>
 __global__ void __launch_bounds__(256)
     serve_kernel(A* a, const serve_req* reqs, uint64_t n, serve_state* st, uint64_t* checksums,
                  uint64_t* admit_ticks, uint64_t* done_ticks, unsigned long long* hist_out, uint64_t headroom,
-                 uint64_t deadline) {
+                 uint64_t deadline, uint32_t run) {
   __shared__ uint32_t hist[kHistBuckets];
   for (uint32_t i = threadIdx.x; i < kHistBuckets; i += blockDim.x) hist[i] = 0;
   __syncthreads();
 
   const uint32_t lane = wave::lane_id();
   const uint64_t t0 = atom::load(&st->t0);
+  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
   uint32_t* free_segs = &a->control[ga::kCtlFreeSegs];
 
   uint32_t state = kSeqIdle;
   uint64_t ticket = 0;
   serve_req r{0, 0, 0, 0};
-  uint32_t step = 0;            // pages allocated (growing) / left (draining)
+  uint64_t stride = 0;          // bytes between the pages of one claim (0: one page per claim)
+  uint32_t step = 0;            // pages written so far
+  uint32_t claimed = 0;         // pages claimed so far (written + reserved)
+  char* reserved = nullptr;     // next reserved page
   uint64_t* page = nullptr;     // the sequence's last page
+  uint32_t drained = 0;         // pages drained since the last claim mark
+  uint64_t* pend_p = nullptr;   // a drained claim waiting to be freed: its first page ...
+  uint32_t pend_k = 0;          // ... and its page count (0: none)
+  bool mark = false;            // the next page written is the first of a claim
   uint64_t h = 0;               // checksum
   uint64_t unflushed = 0;       // allocated bytes not yet taken off `committed`
   bool cut = false;             // aborted by the deadline
-  uint32_t nm = 0, nf = 0, nr = 0, nd = 0, na = 0;
+  uint32_t nm = 0, nf = 0, nr = 0, nd = 0, na = 0, nref = 0;
 
   for (uint32_t it = 0;; it++) {
     const uint64_t now = wall_ticks() - t0;
@@ -112,7 +167,7 @@ __global__ void __launch_bounds__(256)
     if (idle) {
       const int leader = wave::first_lane(idle);
       uint64_t base = 0;
-      if ((int)lane == leader) base = atom::add(&st->cursor, (unsigned long long)wave::popc(idle));
+      if ((int)lane == leader) base = atom::add(&st->cursor.v, (unsigned long long)wave::popc(idle));
       base = wave::readlane64(base, leader);
       if (state == kSeqIdle) {
         ticket = base + wave::popc(idle & wave::lanemask_lt());
@@ -133,10 +188,19 @@ __global__ void __launch_bounds__(256)
       na++;
       state = kSeqIdle;
     }
-    if (late && state == kSeqGrowing) {
+    const bool cutting = late && state == kSeqGrowing;
+    if (cutting) {
       cut = true;
-      unflushed += (uint64_t)(r.pages - step) * r.page_bytes;  // never allocated
+      unflushed += (uint64_t)(r.pages - claimed) * r.page_bytes;  // never claimed
       state = kSeqDraining;
+    }
+    // reserved pages of a cut sequence go back at once
+    const bool spare = cutting && claimed > step;
+    if (wave::ballot(spare)) {
+      if (spare) {
+        a->free_run(reserved, claimed - step);
+        nf += claimed - step;
+      }
     }
 
     // ---- admission of arrived requests, all of the wave's at once when they fit
@@ -148,78 +212,107 @@ __global__ void __launch_bounds__(256)
       const uint64_t total = wave_sum64(need);
       const int leader = wave::first_lane(want);
       if ((int)lane == leader) {
+        // both loads in flight at once; the add only when the ask looks like it fits
         const uint64_t room = (uint64_t)atom::load(free_segs) * A::kBytesPerSegment;
-        uint64_t c = atom::load(&st->committed);
+        const uint64_t seen = atom::load(&st->committed.v);
         // all arrived lanes, else the leader alone
         for (int pass = 0; pass < 2 && !granted; pass++) {
           const uint64_t ask = pass == 0 ? total : need;
           if (pass == 1 && ask == total) break;
-          if (c + ask + headroom > room) continue;
-          c = atom::add(&st->committed, (unsigned long long)ask);
+          if (seen + ask + headroom > room) continue;
+          const uint64_t c = atom::add(&st->committed.v, (unsigned long long)ask);
           if (c + ask + headroom <= room) {
             granted = pass == 0 ? want : (1ull << lane);
-            __hip_atomic_fetch_max(&st->peak_committed, (unsigned long long)(c + ask), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            (void)__hip_atomic_fetch_max(&st->peak_committed.v, (unsigned long long)(c + ask), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
           } else {
-            atom::sub(&st->committed, (unsigned long long)ask);
+            atom::sub(&st->committed.v, (unsigned long long)ask);
           }
         }
-        if (!granted) atom::add(&st->refused, 1ull);
-        if (granted) {
-          const unsigned long long k = wave::popc(granted);
-          const unsigned long long before = atom::add(&st->active, k);
-          __hip_atomic_fetch_max(&st->peak_active, before + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        if (!granted) nref++;
       }
       granted = wave::readlane64(granted, leader);
       if ((granted >> lane) & 1ull) {
         state = kSeqGrowing;
-        step = 0;
+        step = claimed = 0;
         page = nullptr;
         h = r.id;
         cut = false;
+        stride = run > 1 ? A::run_stride(r.page_bytes) : 0;
         admit_ticks[ticket] = now;
       }
     }
 
-    // ---- growing lanes: one page each
+    // ---- growing lanes: one page each, claimed `run` at a time
     const bool grow = state == kSeqGrowing;
-    const uint64_t growing = wave::ballot(grow);
-    if (growing) {
+    const bool claim = grow && claimed == step;
+    const uint64_t claiming = wave::ballot(claim);
+    if (claiming) {
       const uint64_t ta = wall_ticks();
-      uint64_t* p = nullptr;
-      if (grow) p = (uint64_t*)a->malloc(r.page_bytes);
+      uint32_t k = 0;
+      if (claim) {
+        const uint32_t left = r.pages - step;
+        k = stride ? (left < run ? left : run) : 1u;
+        reserved = k > 1 ? (char*)a->malloc_run_at(r.page_bytes, k, tid, __smid(), 0)
+                         : (char*)a->malloc(r.page_bytes);
+      }
       const uint64_t tb = wall_ticks();
-      if ((int)lane == wave::first_lane(growing)) atomicAdd(&hist[hist_bucket(tb - ta)], wave::popc(growing));
-      if (grow) {
-        if (p) {
-          const uint32_t w = gw::page_words(r.page_bytes);
-          p[0] = (uint64_t)page;
-          for (uint32_t k = 1; k < w; k++) p[k] = gw::page_word(r.id, step, k);
-          if (page)
-            for (uint32_t k = 1; k < w; k++) h = gw::fold(h, page[k]);
-          page = p;
-          step++;
-          nm++;
-          unflushed += r.page_bytes;
-          if (step == r.pages) state = kSeqDraining;
+      const uint32_t pages_claimed = (uint32_t)wave_sum64(claim && reserved ? k : 0u);
+      if (lane == 0 && pages_claimed) atomicAdd(&hist[hist_bucket(tb - ta)], pages_claimed);
+      if (claim) {
+        if (reserved) {
+          mark = true;
+          claimed += k;
+          nm += k;
+          unflushed += (uint64_t)k * r.page_bytes;
         } else {
           nr++;
         }
       }
     }
+    if (grow && claimed > step) {
+      uint64_t* p = (uint64_t*)reserved;
+      reserved += stride;
+      const uint32_t w = gw::page_words(r.page_bytes);
+      write_page(p, (uint64_t)page | (mark ? kRunStart : 0ull), r.id, step, w);
+      mark = false;
+      if (page) (void)read_page(page, w, h);
+      page = p;
+      step++;
+      if (step == r.pages) state = kSeqDraining;
+    }
 
-    // ---- draining lanes: one page each, newest first
+    // ---- draining lanes: one page each, newest first; a claim's pages are
+    // released together once its first page is reached. A release waits in
+    // the lane's pending slot; the wave frees every pending claim in one call
+    // (one release fence) once kFreeBatch lanes hold one, or a lane needs its
+    // slot for the next release, or the wave leaves.
     const bool drain = state == kSeqDraining && page != nullptr;
     const uint64_t draining = wave::ballot(drain);
+    bool release = false;
+    uint64_t* first = nullptr;
     if (draining) {
       if (drain) {
-        uint64_t* p = page;
-        const uint32_t w = gw::page_words(r.page_bytes);
-        for (uint32_t k = 1; k < w; k++) h = gw::fold(h, p[k]);
-        page = (uint64_t*)p[0];
-        a->free(p);
-        nf++;
+        first = page;
+        const uint64_t link = read_page(first, gw::page_words(r.page_bytes), h);
+        page = (uint64_t*)(link & ~kRunStart);
+        drained++;
+        release = (link & kRunStart) != 0;
+      }
+      if (wave::ballot(release && pend_k != 0) || wave::popc(wave::ballot(pend_k != 0)) >= kFreeBatch) {
+        if (pend_k) {
+          if (pend_k > 1)
+            a->free_run(pend_p, pend_k);
+          else
+            a->free(pend_p);
+          nf += pend_k;
+          pend_k = 0;
+        }
+      }
+      if (release) {
+        pend_p = first;
+        pend_k = drained;
+        drained = 0;
       }
     }
     // finished (also a cut sequence that never got a page)
@@ -237,30 +330,39 @@ __global__ void __launch_bounds__(256)
         }
         state = kSeqIdle;
       }
-      if ((int)lane == wave::first_lane(finishing))
-        atom::sub(&st->active, (unsigned long long)wave::popc(finishing));
     }
 
     // ---- return allocated bytes to the admission budget, leave when done
     const bool all_done = wave::ballot(state != kSeqExhausted) == 0;
+    if (all_done && wave::ballot(pend_k != 0)) {
+      if (pend_k) {
+        if (pend_k > 1)
+          a->free_run(pend_p, pend_k);
+        else
+          a->free(pend_p);
+        nf += pend_k;
+        pend_k = 0;
+      }
+    }
     if ((it % kFlushEvery) == kFlushEvery - 1 || all_done) {
       const uint64_t s = wave_sum64(unflushed);
-      if (lane == 0 && s) atom::sub(&st->committed, (unsigned long long)s);
+      if (lane == 0 && s) atom::sub(&st->committed.v, (unsigned long long)s);
       unflushed = 0;
     }
     if (all_done) break;
-    if (!growing && !draining && !granted) __builtin_amdgcn_s_sleep(32);
+    if (!wave::ballot(grow) && !draining && !granted) __builtin_amdgcn_s_sleep(32);
   }
 
   // per-wave totals
   const uint64_t tm = wave_sum64(nm), tf = wave_sum64(nf), tr = wave_sum64(nr);
-  const uint64_t td = wave_sum64(nd), ta = wave_sum64(na);
+  const uint64_t td = wave_sum64(nd), ta = wave_sum64(na), tref = wave_sum64(nref);
   if (lane == 0) {
     if (tm) atom::add(&st->page_mallocs, (unsigned long long)tm);
     if (tf) atom::add(&st->page_frees, (unsigned long long)tf);
     if (tr) atom::add(&st->page_retries, (unsigned long long)tr);
     if (td) atom::add(&st->done, (unsigned long long)td);
     if (ta) atom::add(&st->aborted, (unsigned long long)ta);
+    if (tref) atom::add(&st->refused, (unsigned long long)tref);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < kHistBuckets; i += blockDim.x)

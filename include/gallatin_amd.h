@@ -285,11 +285,12 @@ typedef struct gal_serve_result {
   uint64_t peak_committed;     /* most admitted-but-unallocated page bytes */
   uint64_t alloc_errors;       /* growth of the allocator's internal error count */
   uint64_t slots;              /* sequence slots launched */
+  uint64_t run;                /* pages claimed per allocator call (at most) */
   uint64_t headroom;           /* headroom bytes used by admission */
   double kernel_ms;            /* the serving launch (HIP events on the handle's stream) */
   double makespan_ms;          /* start of the loop to the last completion (device clock) */
-  double page_malloc_p50_ns;   /* per-page malloc latency (wave time inside malloc, */
-  double page_malloc_p99_ns;   /*   one sample per page, 1/16-octave histogram) */
+  double page_malloc_p50_ns;   /* per-page malloc latency: the wave's time inside the allocator */
+  double page_malloc_p99_ns;   /*   call that claimed the page, one sample per page (1/16 octave) */
   double page_malloc_max_ns;
   double queue_p50_ns;         /* arrival -> admission */
   double queue_p99_ns;
@@ -302,7 +303,11 @@ int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out);
 int gal_serve_destroy(gal_serve* s);
 int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n);
 /* Serve the loaded requests in one persistent launch of `slots` sequence
- * slots (rounded up to 256, capped at what the GPU holds at once). headroom
+ * slots (rounded up to 256, capped at what the GPU holds at once). A growing
+ * sequence claims its next min(run, pages left) pages (run in 1..64) with one
+ * bulk claim (consecutive slices of one block, each page still a separate
+ * allocation) and frees them with one free_run once they are all read back;
+ * run 1 is one malloc and one free per page. headroom
  * GAL_SERVE_DEFAULT_HEADROOM: for each class the page sizes claim from, its
  * pinned and overflow slots' blocks and two segments; plus a tenth of the
  * pool when such a class has several blocks per segment (recycled blocks of
@@ -310,7 +315,8 @@ int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n);
  * gallatin.cuh:1646). Requests not finished deadline_s
  * seconds after the start are aborted (their pages are freed). Returns
  * GAL_ERR_INVALID if a request could never be admitted. */
-int gal_serve_run(gal_serve* s, uint32_t slots, uint64_t headroom, double deadline_s, gal_serve_result* out);
+int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s,
+                  gal_serve_result* out);
 /* per request of the last run: checksum (0 if aborted), admission and
  * completion times in ns after the start (~0 if aborted) */
 int gal_serve_results(gal_serve* s, uint64_t* checksums, uint64_t* admit_ns, uint64_t* done_ns, uint64_t cap);

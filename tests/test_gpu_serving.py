@@ -57,16 +57,17 @@ def test_free_gauge_tracks_segment_tree():
         g.free_on_device()
 
 
-def test_serving_mixed_sizes_alone_equals_mix():
+@pytest.mark.parametrize("run", [1, 16])
+def test_serving_mixed_sizes_alone_equals_mix(run):
     pool = 8 << 30
     reqs = ga.generate_requests(4000, 0.0, median_pages=48, sigma=1.0, max_pages=2048, page_bytes=0,
                                 size_classes=9, seed=31)
-    say(f"[serve-mix] n={len(reqs)} pages={int(reqs['pages'].sum())}")
+    say(f"[serve-mix] run={run} n={len(reqs)} pages={int(reqs['pages'].sum())}")
     g = ga.Gallatin.generate_on_device(pool, 12)
     try:
         with ga.Serving(g, len(reqs)) as s:
             s.load(reqs)
-            r = s.run(slots=8192, deadline_s=20.0)
+            r = s.run(slots=8192, run=run, deadline_s=20.0)
             say(f"  mix {r}")
             cs, admit, done = s.results()
             check_run(r, reqs, cs)
@@ -77,7 +78,7 @@ def test_serving_mixed_sizes_alone_equals_mix():
             for j in picks:
                 one = reqs[j : j + 1].copy()
                 s.load(one)
-                r1 = s.run(slots=256, deadline_s=10.0)
+                r1 = s.run(slots=256, run=1 + (j % 7) * run // 4, deadline_s=10.0)
                 cs1, _, _ = s.results()
                 say(f"  alone {j}: pages {int(one['pages'][0])} x {int(one['page_bytes'][0])} B -> {r1['requests_done']}")
                 assert r1["requests_done"] == 1 and int(cs1[0]) == int(cs[j]), (j, int(cs1[0]), int(cs[j]))
@@ -86,7 +87,8 @@ def test_serving_mixed_sizes_alone_equals_mix():
         g.free_on_device()
 
 
-def test_serving_admission_gates_on_free_bytes():
+@pytest.mark.parametrize("run", [1, 32])
+def test_serving_admission_gates_on_free_bytes(run):
     pool = 2 << 30
     reqs = ga.generate_requests(300, 0.0, median_pages=512, sigma=1.0, max_pages=4096, page_bytes=16384, seed=41)
     demand = int(reqs["pages"].astype(np.int64).sum()) * 16384
@@ -97,8 +99,8 @@ def test_serving_admission_gates_on_free_bytes():
         assert demand > free0
         with ga.Serving(g, len(reqs)) as s:
             s.load(reqs)
-            r = s.run(slots=4096, deadline_s=20.0)
-            say(f"  {r}")
+            r = s.run(slots=4096, run=run, deadline_s=20.0)
+            say(f"  run={run} {r}")
             cs, admit, done = s.results()
             check_run(r, reqs, cs)
             assert r["admission_refusals"] > 0, "admission never had to wait"
@@ -137,15 +139,16 @@ def test_serving_poisson_arrivals_are_respected_and_overlap():
         g.free_on_device()
 
 
-def test_serving_deadline_aborts_cleanly():
+@pytest.mark.parametrize("run", [1, 8])
+def test_serving_deadline_aborts_cleanly(run):
     pool = 4 << 30
     reqs = ga.generate_requests(2000, 0.0, median_pages=4096, sigma=0.5, max_pages=32768, page_bytes=16384, seed=61)
     g = ga.Gallatin.generate_on_device(pool, 15)
     try:
         with ga.Serving(g, len(reqs)) as s:
             s.load(reqs)
-            r = s.run(slots=1024, deadline_s=0.002)
-            say(f"[serve-deadline] {r}")
+            r = s.run(slots=1024, run=run, deadline_s=0.002)
+            say(f"[serve-deadline] run={run} {r}")
             cs, admit, done = s.results()
             assert r["requests_done"] + r["requests_aborted"] == len(reqs)
             assert r["requests_aborted"] > 0
