@@ -218,6 +218,31 @@ class Gallatin:
         )
         return buf[: n.value].astype(np.float64) * tick.value
 
+    def churn(self, kinds, args, seed=0):
+        """One concurrent kernel: op i is malloc(args[i]) (kind GAL_OP_MALLOC) or
+        free(pool + args[i]) (GAL_OP_FREE). Returns (results, run result)."""
+        kinds = np.asarray(kinds, dtype=np.uint32)
+        args = np.asarray(args, dtype=np.uint64)
+        n = len(kinds)
+        arr = (GalOp * max(1, n))()
+        view = np.frombuffer(arr, dtype=np.dtype([("kind", "<u4"), ("slot", "<u4"), ("arg", "<u8"), ("tid", "<u8")]),
+                             count=n)
+        view["kind"] = kinds
+        view["arg"] = args
+        out = np.zeros(max(1, n), dtype=np.uint64)
+        r = _capi.GalRunResult()
+        _capi.check(
+            "gal_churn",
+            _lib.gal_churn(self._h, arr, n, seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(r)),
+        )
+        return out[:n], r.as_dict()
+
+    def largest_segment_run(self):
+        """Most contiguous segments one malloc could claim now (gal_largest_run)."""
+        out = ctypes.c_uint64()
+        _capi.check("gal_largest_run", _lib.gal_largest_run(self._h, ctypes.byref(out)))
+        return out.value
+
     def free_bytes(self):
         """Bytes in free segments (the allocator's gauge, gal_free_bytes)."""
         out = ctypes.c_uint64()
@@ -318,6 +343,16 @@ class Batch:
         p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         _capi.check("gal_batch_order", _lib.gal_batch_order(self._b, p, self.n))
         return out[: self.n]
+
+
+def gather_limit(words):
+    """Most contiguous items the segment tree's gather rule could claim from
+    these item-layer words (gal_gather_limit; host only)."""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    out = ctypes.c_uint64()
+    _capi.check("gal_gather_limit", _lib.gal_gather_limit(w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(w),
+                                                          ctypes.byref(out)))
+    return out.value
 
 
 # ---- serving loop (no reference counterpart; every page is a Gallatin malloc/free)

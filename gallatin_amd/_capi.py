@@ -168,6 +168,9 @@ _SIGNATURES = {
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],
     ),
     "gal_free_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "gal_churn": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, u64, ctypes.POINTER(u64), ctypes.POINTER(GalRunResult)]),
+    "gal_gather_limit": (ctypes.c_int, [ctypes.POINTER(u64), u64, ctypes.POINTER(u64)]),
+    "gal_largest_run": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "gal_serve_generate": (ctypes.c_int, [ctypes.POINTER(GalServeWorkload), vp]),
     "gal_serve_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
     "gal_serve_destroy": (ctypes.c_int, [vp]),

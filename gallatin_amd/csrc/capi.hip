@@ -106,6 +106,8 @@ struct impl_ops {
   void (*batch_bin)(gal_batch* b);
   void (*batch_pass)(gal_batch* b, uint32_t mode, uint32_t width_log2, uint32_t run, uint32_t verify);
   void (*batch_per_thread)(gal_batch* b, uint32_t verify);
+  void (*launch_churn)(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t seed, uint64_t* d_results);
+  int (*segment_words)(gal_allocator* h, std::vector<uint64_t>& words);
   // serving loop (serve.inl)
   uint64_t (*serve_headroom)(gal_allocator* h, const std::vector<uint32_t>& page_sizes);
   unsigned (*serve_resident_blocks)();
@@ -311,6 +313,76 @@ __global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) 
   count(c, got, got, miss, bad);
 }
 
+// Concurrent mixed malloc/free: thread i runs op i (any interleaving with
+// the others). A malloc tags the first word of its allocation with a tag of
+// its offset (atomic exchange, expects 0); a free checks and clears it first.
+template <class A>
+__global__ void churn_kernel(A* a, const gal_op* ops, uint64_t n, uint64_t seed, uint64_t* results,
+                             run_counters* c) {
+  const uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool got = false, freed = false, miss = false, bad = false;
+  if (i < n) {
+    const gal_op op = ops[i];
+    char* base = a->table.memory;
+    if (op.kind == GAL_OP_MALLOC) {
+      uint64_t* p = (uint64_t*)a->malloc(op.arg);
+      if (p) {
+        const uint64_t off = (uint64_t)((char*)p - base);
+        bad = !tag_in(p, gw::payload_tag(seed, off), GAL_VERIFY_ATOMIC);
+        results[i] = off;
+        got = true;
+      } else {
+        results[i] = GAL_NULL_OFFSET;
+        miss = true;
+      }
+    } else {
+      uint64_t* p = (uint64_t*)(base + op.arg);
+      bad = !tag_out(p, gw::payload_tag(seed, op.arg), GAL_VERIFY_ATOMIC);
+      a->free(p);
+      results[i] = 0;
+      freed = true;
+    }
+  }
+  count(c, got, freed, miss, bad);
+}
+
+// The segment tree's gather rule (Gallatin::malloc_segment_allocation ->
+// veb_tree::gather_bottom, reference veb.cuh:323-452) evaluated without
+// claiming anything: can n contiguous segments be claimed from these item
+// words right now? Same control flow as the claim: words from the top down,
+// a run is a word's trailing ones extended through full words below and the
+// top bits of the next lower one; it may not reach below word 0.
+bool gather_would_succeed(const uint64_t* b, uint64_t words, uint64_t n) {
+  if (n == 0 || words == 0) return false;
+  uint64_t w = words - 1;
+  while (w != 0) {
+    uint64_t need = n;
+    const int run = gu::trailing_ones(b[w]);
+    if (run == 0) {
+      w--;
+      continue;
+    }
+    if ((uint64_t)run >= need) return true;
+    need -= (uint64_t)run;
+    bool ok = true;
+    while (need > 64) {
+      if (w == 0) return false;
+      w--;
+      if (b[w] != ~0ull) {
+        ok = false;
+        break;
+      }
+      need -= 64;
+    }
+    if (!ok) continue;
+    if (w == 0) return false;
+    w--;
+    const uint64_t top = gu::bitmask(need) << (64 - need);
+    if ((b[w] & top) == top) return true;
+  }
+  return false;
+}
+
 inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
 #include "batch.inl"
@@ -426,6 +498,19 @@ struct impl {
                            0, h->stream>>>((A*)h->dev, h->d_live, n_live, round, seed, verify, all, h->d_counters);
   }
   static void print(gal_allocator* h) { ((A*)h->dev)->print_info(); }
+  static void launch_churn(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t seed, uint64_t* d_results) {
+    churn_kernel<A><<<blocks_for(n, 256), 256, 0, h->stream>>>((A*)h->dev, d_ops, n, seed, d_results, h->d_counters);
+  }
+  // item words of the segment tree
+  static int segment_words(gal_allocator* h, std::vector<uint64_t>& words) {
+    A host;
+    if (!A::host_copy((const A*)h->dev, &host)) return GAL_ERR_HIP;
+    const ga::veb_tree& t = host.segment_tree;
+    words.resize(t.words[t.num_layers - 1]);
+    return hipMemcpy(words.data(), t.bits[t.num_layers - 1], words.size() * 8, hipMemcpyDeviceToHost) == hipSuccess
+               ? GAL_OK
+               : GAL_ERR_HIP;
+  }
 
   static void batch_bin(gal_batch* b) {
     const uint64_t waves = (b->n + kBinWaveRequests - 1) / kBinWaveRequests;
@@ -515,8 +600,8 @@ struct impl {
   static constexpr impl_ops ops() {
     return impl_ops{S,         M,          B,       &create,       &destroy,           &describe,
                     &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
-                    &batch_bin, &batch_pass, &batch_per_thread, &serve_headroom, &serve_resident_blocks,
-                    &serve_launch};
+                    &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, &segment_words, &serve_headroom,
+                    &serve_resident_blocks, &serve_launch};
   }
 };
 
@@ -1044,6 +1129,53 @@ int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap) {
   if (!b || !order || cap < b->n) return GAL_ERR_INVALID;
   if (hipStreamSynchronize(b->h->stream) != hipSuccess) return GAL_ERR_HIP;
   return hipMemcpy(order, b->d_perm, b->n * 4, hipMemcpyDeviceToHost) == hipSuccess ? GAL_OK : GAL_ERR_HIP;
+}
+
+int gal_churn(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t seed, uint64_t* results,
+              gal_run_result* out) {
+  if (!h || !ops || !results || n == 0) return GAL_ERR_INVALID;
+  const uint64_t pool = h->info.num_segments * h->info.segment_size;
+  for (uint64_t i = 0; i < n; i++) {
+    if (ops[i].kind == GAL_OP_MALLOC) {
+      if (ops[i].arg == 0) return GAL_ERR_INVALID;
+    } else if (ops[i].kind != GAL_OP_FREE || ops[i].arg >= pool || (ops[i].arg & 7) != 0) {
+      return GAL_ERR_INVALID;
+    }
+  }
+  gal_op* d_ops = nullptr;
+  uint64_t* d_res = nullptr;
+  int rc = GAL_OK;
+  if (hipMalloc(&d_ops, n * sizeof(gal_op)) != hipSuccess || hipMalloc(&d_res, n * 8) != hipSuccess) rc = GAL_ERR_OOM;
+  if (rc == GAL_OK && hipMemcpy(d_ops, ops, n * sizeof(gal_op), hipMemcpyHostToDevice) != hipSuccess) rc = GAL_ERR_HIP;
+  if (rc == GAL_OK)
+    rc = timed_run(h, out, [&](launch_timer& t) {
+      if (!t.begin()) return GAL_ERR_INVALID;
+      h->ops->launch_churn(h, d_ops, n, seed, d_res);
+      t.end(0);
+      return GAL_OK;
+    });
+  if (rc == GAL_OK && hipMemcpy(results, d_res, n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = GAL_ERR_HIP;
+  if (d_ops) (void)hipFree(d_ops);
+  if (d_res) (void)hipFree(d_res);
+  return rc;
+}
+
+int gal_gather_limit(const uint64_t* words, uint64_t nwords, uint64_t* segments) {
+  if (!words || !segments || nwords == 0) return GAL_ERR_INVALID;
+  uint64_t best = 0;
+  for (uint64_t n = 1; n <= nwords * 64; n++)
+    if (gather_would_succeed(words, nwords, n)) best = n;
+  *segments = best;
+  return GAL_OK;
+}
+
+int gal_largest_run(gal_allocator* h, uint64_t* segments) {
+  if (!h || !segments) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  std::vector<uint64_t> words;
+  const int rc = h->ops->segment_words(h, words);
+  if (rc != GAL_OK) return rc;
+  return gal_gather_limit(words.data(), words.size(), segments);
 }
 
 int gal_free_bytes(gal_allocator* h, uint64_t* bytes) {

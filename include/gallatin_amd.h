@@ -229,6 +229,30 @@ int gal_batch_offsets(gal_batch* b, uint64_t* offsets, uint64_t cap);
 /* request indices in the binned order of the last binning */
 int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap);
 
+/* ---- concurrent churn ----
+ * One kernel, one thread per op, all ops of the call running concurrently
+ * in any interleaving (the reference's test kernels mix mallocs and frees
+ * the same way, tests/src/gallatin_test.cu:58-219):
+ *   GAL_OP_MALLOC: arg = size; results[i] = byte offset from the pool base,
+ *                  or GAL_NULL_OFFSET when malloc returned nullptr
+ *   GAL_OP_FREE:   arg = byte offset of a live allocation (made by an
+ *                  earlier call); results[i] = 0
+ * slot and tid are ignored (the kernel uses the hardware CU id and its own
+ * thread ids). Each malloc tags its allocation's first word with a tag of
+ * (seed, offset) by atomic exchange (expects 0) and each free checks and
+ * clears it: out->tag_errors counts mismatches (a shared allocation). */
+int gal_churn(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t seed, uint64_t* results,
+              gal_run_result* out);
+
+/* Largest number of contiguous segments the segment tree's gather rule
+ * (reference veb.cuh:323-452, used by malloc for requests above one block)
+ * could claim right now: the fragmentation of the multi-segment path.
+ * gal_gather_limit evaluates the rule on caller-supplied item words (host
+ * only); gal_largest_run on the allocator's segment tree. No counterpart
+ * in the reference (its print_info reports fill, gallatin.cuh:1855-1897). */
+int gal_gather_limit(const uint64_t* words, uint64_t nwords, uint64_t* segments);
+int gal_largest_run(gal_allocator* h, uint64_t* segments);
+
 /* Bytes in free segments (segments in the segment tree) right now: the
  * allocator's own gauge, kept exact at every segment claim and return. No
  * counterpart in the reference (its print_info counts the tree's bits,
