@@ -419,6 +419,42 @@ class Serving:
         return tuple(a[: self.n] for a in out)
 
 
+# ---- slab hash client (reference data_structs/chaining_table.cuh)
+class SlabTable:
+    """Chained slab hash table whose 120-byte slabs come from the allocator
+    (gal_slab_* in include/gallatin_amd.h)."""
+
+    def __init__(self, allocator, buckets, seed=123456):
+        h = ctypes.c_void_p()
+        _capi.check("gal_slab_create", _lib.gal_slab_create(allocator._h, buckets, seed, ctypes.byref(h)))
+        self._t = h
+        self.allocator = allocator
+        self.buckets = buckets
+
+    def insert(self, n, key_seed):
+        r = _capi.GalSlabResult()
+        _capi.check("gal_slab_insert", _lib.gal_slab_insert(self._t, n, key_seed, ctypes.byref(r)))
+        return r.as_dict()
+
+    def query(self, n, key_seed):
+        r = _capi.GalSlabResult()
+        _capi.check("gal_slab_query", _lib.gal_slab_query(self._t, n, key_seed, ctypes.byref(r)))
+        return r.as_dict()
+
+    def destroy(self):
+        freed = ctypes.c_uint64()
+        if self._t:
+            _capi.check("gal_slab_destroy", _lib.gal_slab_destroy(self._t, ctypes.byref(freed)))
+            self._t = None
+        return freed.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
 # ---- device atomic rates (the bound of the slice path; no reference counterpart)
 ATOMIC_DISTINCT = 0
 ATOMIC_WAVE_HOT = 1

@@ -126,6 +126,21 @@ class GalServeResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GalSlabResult(ctypes.Structure):
+    _fields_ = [
+        ("keys", u64),
+        ("failed", u64),
+        ("wrong", u64),
+        ("slabs", u64),
+        ("max_chain", u64),
+        ("alloc_errors", u64),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 GAL_SERVE_DEFAULT_HEADROOM = (1 << 64) - 1
 
 _SIGNATURES = {
@@ -168,6 +183,10 @@ _SIGNATURES = {
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],
     ),
     "gal_free_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "gal_slab_create": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(vp)]),
+    "gal_slab_insert": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(GalSlabResult)]),
+    "gal_slab_query": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(GalSlabResult)]),
+    "gal_slab_destroy": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "gal_churn": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, u64, ctypes.POINTER(u64), ctypes.POINTER(GalRunResult)]),
     "gal_gather_limit": (ctypes.c_int, [ctypes.POINTER(u64), u64, ctypes.POINTER(u64)]),
     "gal_largest_run": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),

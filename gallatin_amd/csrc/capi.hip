@@ -12,6 +12,7 @@
 #include <vector>
 
 #include <gallatin/allocators/global_allocator.hpp>
+#include <gallatin/data_structs/chaining_table.hpp>
 #include <gallatin/workloads.hpp>
 
 #include "../../include/gallatin_amd.h"
@@ -70,6 +71,15 @@ struct gal_batch {
   uint64_t** d_ptrs = nullptr;    // results of gal_batch_malloc
 };
 
+// a slab hash table over the handle's allocator
+struct gal_slab_table {
+  gal_allocator* h = nullptr;
+  void* dev = nullptr;     // chaining_table<u64, u64, 7, A> in device memory
+  void* heads = nullptr;   // bucket heads
+  uint64_t buckets = 0;
+  void* counters = nullptr;  // slab_counters + 2 chain words
+};
+
 // a loaded serving workload and the device buffers of its runs
 struct gal_serve {
   gal_allocator* h = nullptr;
@@ -107,6 +117,10 @@ struct impl_ops {
   void (*batch_pass)(gal_batch* b, uint32_t mode, uint32_t width_log2, uint32_t run, uint32_t verify);
   void (*batch_per_thread)(gal_batch* b, uint32_t verify);
   void (*launch_churn)(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t seed, uint64_t* d_results);
+  // slab hash client (slab.inl)
+  size_t slab_table_bytes;
+  void (*slab_init)(gal_slab_table* t, uint64_t seed);
+  void (*slab_launch)(gal_slab_table* t, int what, uint64_t n, uint64_t key_seed);
   int (*segment_words)(gal_allocator* h, std::vector<uint64_t>& words);
   // serving loop (serve.inl)
   uint64_t (*serve_headroom)(gal_allocator* h, const std::vector<uint32_t>& page_sizes);
@@ -387,6 +401,7 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n 
 
 #include "batch.inl"
 #include "serve.inl"
+#include "slab.inl"
 
 // ------------------------------------------------------ per-type impl
 template <uint64_t S, uint64_t M, uint64_t B>
@@ -501,6 +516,28 @@ struct impl {
   static void launch_churn(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t seed, uint64_t* d_results) {
     churn_kernel<A><<<blocks_for(n, 256), 256, 0, h->stream>>>((A*)h->dev, d_ops, n, seed, d_results, h->d_counters);
   }
+  using table_t = gallatin::data_structs::chaining_table<uint64_t, uint64_t, kSlabSlots, A>;
+  static void slab_init(gal_slab_table* t, uint64_t seed) {
+    table_t host;
+    host.heads = (typename table_t::block_type**)t->heads;
+    host.nbuckets = t->buckets;
+    host.seed = seed;
+    host.empty = 0;
+    host.alloc = (A*)t->h->dev;
+    (void)hipMemcpy(t->dev, &host, sizeof(host), hipMemcpyHostToDevice);
+  }
+  // what: 0 insert, 1 query, 2 chain lengths, 3 free every slab
+  static void slab_launch(gal_slab_table* t, int what, uint64_t n, uint64_t key_seed) {
+    table_t* d = (table_t*)t->dev;
+    slab_counters* c = (slab_counters*)t->counters;
+    unsigned long long* chains = (unsigned long long*)((char*)t->counters + sizeof(slab_counters));
+    hipStream_t st = t->h->stream;
+    if (what == 0) slab_insert_kernel<table_t><<<blocks_for(n, 256), 256, 0, st>>>(d, n, key_seed, c);
+    if (what == 1) slab_query_kernel<table_t><<<blocks_for(n, 256), 256, 0, st>>>(d, n, key_seed, c);
+    if (what == 2) slab_chain_kernel<table_t><<<blocks_for(t->buckets, 256), 256, 0, st>>>(d, chains);
+    if (what == 3) slab_free_kernel<table_t><<<blocks_for(t->buckets, 256), 256, 0, st>>>(d, chains);
+  }
+
   // item words of the segment tree
   static int segment_words(gal_allocator* h, std::vector<uint64_t>& words) {
     A host;
@@ -600,8 +637,8 @@ struct impl {
   static constexpr impl_ops ops() {
     return impl_ops{S,         M,          B,       &create,       &destroy,           &describe,
                     &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
-                    &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, &segment_words, &serve_headroom,
-                    &serve_resident_blocks, &serve_launch};
+                    &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, sizeof(table_t), &slab_init,
+                    &slab_launch, &segment_words, &serve_headroom, &serve_resident_blocks, &serve_launch};
   }
 };
 
@@ -1176,6 +1213,89 @@ int gal_largest_run(gal_allocator* h, uint64_t* segments) {
   const int rc = h->ops->segment_words(h, words);
   if (rc != GAL_OK) return rc;
   return gal_gather_limit(words.data(), words.size(), segments);
+}
+
+// ---- slab hash client
+int gal_slab_create(gal_allocator* h, uint64_t buckets, uint64_t seed, gal_slab_table** out) {
+  if (!h || !out || buckets == 0) return GAL_ERR_INVALID;
+  *out = nullptr;
+  gal_slab_table* t = new gal_slab_table();
+  t->h = h;
+  t->buckets = buckets;
+  const size_t ctr = sizeof(slab_counters) + 2 * sizeof(unsigned long long);
+  if (hipMalloc(&t->dev, h->ops->slab_table_bytes) != hipSuccess || hipMalloc(&t->heads, buckets * 8) != hipSuccess ||
+      hipMalloc(&t->counters, ctr) != hipSuccess || hipMemset(t->heads, 0, buckets * 8) != hipSuccess) {
+    if (t->dev) (void)hipFree(t->dev);
+    if (t->heads) (void)hipFree(t->heads);
+    if (t->counters) (void)hipFree(t->counters);
+    delete t;
+    return GAL_ERR_OOM;
+  }
+  h->ops->slab_init(t, seed);
+  *out = t;
+  return GAL_OK;
+}
+
+// run one slab kernel with zeroed counters; fills the result's counts
+static int slab_run(gal_slab_table* t, int what, uint64_t n, uint64_t key_seed, gal_slab_result* out) {
+  gal_allocator* h = t->h;
+  const size_t ctr = sizeof(slab_counters) + 2 * sizeof(unsigned long long);
+  gal_run_result base;
+  std::memset(&base, 0, sizeof(base));
+  int rc = timed_run(h, &base, [&](launch_timer& tm) {
+    if (hipMemsetAsync(t->counters, 0, ctr, h->stream) != hipSuccess) return GAL_ERR_HIP;
+    if (!tm.begin()) return GAL_ERR_INVALID;
+    h->ops->slab_launch(t, what, n, key_seed);
+    tm.end(0);
+    return GAL_OK;
+  });
+  if (rc != GAL_OK) return rc;
+  unsigned char buf[sizeof(slab_counters) + 16];
+  if (hipMemcpy(buf, t->counters, ctr, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  slab_counters c;
+  unsigned long long chains[2];
+  std::memcpy(&c, buf, sizeof(c));
+  std::memcpy(chains, buf + sizeof(c), 16);
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->keys = n;
+    out->failed = c.failed;
+    out->wrong = c.wrong;
+    out->slabs = chains[0];
+    out->max_chain = chains[1];
+    out->alloc_errors = base.alloc_errors;
+    out->kernel_ms = base.malloc_kernel_ms;
+  }
+  return GAL_OK;
+}
+
+int gal_slab_insert(gal_slab_table* t, uint64_t n, uint64_t key_seed, gal_slab_result* out) {
+  if (!t || n == 0) return GAL_ERR_INVALID;
+  gal_slab_result r, chains;
+  int rc = slab_run(t, 0, n, key_seed, &r);
+  if (rc == GAL_OK) rc = slab_run(t, 2, 0, 0, &chains);
+  if (rc != GAL_OK) return rc;
+  r.slabs = chains.slabs;
+  r.max_chain = chains.max_chain;
+  if (out) *out = r;
+  return GAL_OK;
+}
+
+int gal_slab_query(gal_slab_table* t, uint64_t n, uint64_t key_seed, gal_slab_result* out) {
+  if (!t || n == 0) return GAL_ERR_INVALID;
+  return slab_run(t, 1, n, key_seed, out);
+}
+
+int gal_slab_destroy(gal_slab_table* t, uint64_t* slabs_freed) {
+  if (!t) return GAL_ERR_INVALID;
+  gal_slab_result r;
+  int rc = slab_run(t, 3, 0, 0, &r);
+  if (slabs_freed) *slabs_freed = rc == GAL_OK ? r.slabs : 0;
+  (void)hipFree(t->dev);
+  (void)hipFree(t->heads);
+  (void)hipFree(t->counters);
+  delete t;
+  return rc;
 }
 
 int gal_free_bytes(gal_allocator* h, uint64_t* bytes) {

@@ -253,6 +253,32 @@ int gal_churn(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t seed, ui
 int gal_gather_limit(const uint64_t* words, uint64_t nwords, uint64_t* segments);
 int gal_largest_run(gal_allocator* h, uint64_t* segments);
 
+/* ---- slab hash client ----
+ * A chained hash table of 64-bit keys and values, 7 per 120-byte slab, whose
+ * slabs are malloc'd from the handle's allocator inside the insert kernel:
+ * the allocator's client workload. Replaces the reference's
+ * chaining_table<Key, Val, 7> (include/gallatin/data_structs/
+ * chaining_table.cuh:57-300) as driven by tests/src/chaining_test.cu:80-155;
+ * the device structure is gallatin_amd/include/gallatin/data_structs/
+ * chaining_table.hpp. Key i of a stream is a bijection of (key_seed + i)
+ * (never 0, the empty mark); its value is i. */
+typedef struct gal_slab_table gal_slab_table;
+typedef struct gal_slab_result {
+  uint64_t keys;         /* keys inserted or queried */
+  uint64_t failed;       /* inserts that found no memory; queries that missed */
+  uint64_t wrong;        /* queries that found another value */
+  uint64_t slabs;        /* slabs in the table after an insert (or freed by destroy) */
+  uint64_t max_chain;    /* longest chain after an insert, slabs */
+  uint64_t alloc_errors; /* growth of the allocator's internal error count */
+  double kernel_ms;      /* the insert / query kernel (HIP events, handle's stream) */
+} gal_slab_result;
+int gal_slab_create(gal_allocator* h, uint64_t buckets, uint64_t seed, gal_slab_table** out);
+/* one kernel, one thread per key */
+int gal_slab_insert(gal_slab_table* t, uint64_t n, uint64_t key_seed, gal_slab_result* out);
+int gal_slab_query(gal_slab_table* t, uint64_t n, uint64_t key_seed, gal_slab_result* out);
+/* frees every slab back to the allocator, then the table */
+int gal_slab_destroy(gal_slab_table* t, uint64_t* slabs_freed);
+
 /* Bytes in free segments (segments in the segment tree) right now: the
  * allocator's own gauge, kept exact at every segment claim and return. No
  * counterpart in the reference (its print_info counts the tree's bits,
