@@ -62,6 +62,154 @@ def cpu_baseline(pool, seed, threads, rounds):
     }
 
 
+def rate(ops, ms):
+    return ops / (ms * 1e-3) if ms > 0 else None
+
+
+def forms(seed):
+    """The forms of service on one request mix (16 B-4 KiB, log-uniform powers
+    of two): one thread per request (malloc, tag, check, free), the binned
+    persistent launch, the same launch replayed as a hipGraph, and the
+    serving loop over sequences of pages of those sizes."""
+    import gallatin_amd as ga
+
+    n = 1 << 22
+    out = {"requests": n, "sizes": "16 B-4 KiB, 16 << k, k uniform in [0, 9)"}
+    g = ga.Gallatin.generate_on_device(8 << 30, seed)
+    try:
+        with ga.Batch(g, n) as b:
+            b.generate(n, seed=seed, size_classes=9)
+            for name, form, width, run in (("per_thread", ga.FORM_PER_THREAD, 64, 1),
+                                           ("batched", ga.FORM_BATCHED, 64, 16),
+                                           ("graph", ga.FORM_GRAPH, 64, 16)):
+                b.run(form, width=width, run=run, replays=1)  # warm-up
+                r = b.run(form, width=width, run=run, replays=5, verify=ga.GAL_VERIFY_STORE)
+                out[name] = {"mallocs_frees_per_s": rate(r["mallocs"] + r["frees"], r["total_ms"]),
+                             "misses": r["misses"], "tag_errors": r["tag_errors"], "alloc_errors": r["alloc_errors"],
+                             "width": width, "run": run}
+                log(f"[bench] form {name}: {out[name]}")
+        reqs = ga.generate_requests(1 << 20, 0.0, median_pages=16, sigma=1.0, max_pages=1024, page_bytes=0,
+                                    size_classes=9, seed=seed)
+        with ga.Serving(g, len(reqs)) as s:
+            s.load(reqs)
+            s.run(slots=65536, run=64, deadline_s=60.0)  # warm-up
+            r = s.run(slots=65536, run=64, deadline_s=60.0)
+        out["serving"] = {"mallocs_frees_per_s": rate(r["page_mallocs"] + r["page_frees"], r["kernel_ms"]),
+                          "workload": f"{len(reqs)} sequences all arrived at once, lognormal length (median 16, "
+                          "sigma 1, cap 1024 pages), page sizes as above, 65536 slots, runs of 64",
+                          "page_malloc_p50_ns": r["page_malloc_p50_ns"], "page_malloc_p99_ns": r["page_malloc_p99_ns"],
+                          "aborted": r["requests_aborted"], "alloc_errors": r["alloc_errors"]}
+        log(f"[bench] form serving: {out['serving']}")
+        base = out["per_thread"]["mallocs_frees_per_s"]
+        out["vs_per_thread"] = {k: out[k]["mallocs_frees_per_s"] / base for k in ("batched", "graph", "serving")}
+        out["errors"] = g.errors()
+    finally:
+        g.free_on_device()
+    return out
+
+
+def serving(seed, requests, rate_per_s):
+    """Sequences arriving by a Poisson process, growing by 16 KiB pages, admitted
+    by free pool bytes, over a 200 GiB pool."""
+    import gallatin_amd as ga
+
+    g = ga.Gallatin.generate_on_device(200 << 30, seed)
+    try:
+        reqs = ga.generate_requests(requests, rate_per_s, median_pages=1024, sigma=1.0, max_pages=32768,
+                                    page_bytes=16384, seed=seed)
+        with ga.Serving(g, len(reqs)) as s:
+            s.load(reqs)
+            r = s.run(slots=16384, run=64, deadline_s=float(reqs["arrival_ns"][-1]) * 1e-9 + 60.0)
+        sec = r["kernel_ms"] * 1e-3
+        out = {
+            "workload": f"200 GiB pool, {requests} requests, Poisson {rate_per_s:.0f}/s, lognormal length (median "
+            "1024, sigma 1, cap 32768) of 16 KiB pages, admission by free pool bytes, 16384 slots, runs of 64",
+            "arrival_span_s": float(reqs["arrival_ns"][-1]) * 1e-9,
+            "requests_per_s": r["requests_done"] / sec if sec > 0 else None,
+            "page_mallocs_frees_per_s": rate(r["page_mallocs"] + r["page_frees"], r["kernel_ms"]),
+            "p50_page_malloc_ns": r["page_malloc_p50_ns"],
+            "p99_page_malloc_ns": r["page_malloc_p99_ns"],
+        }
+        out.update({k: r[k] for k in ("requests_done", "requests_aborted", "page_mallocs", "page_retries",
+                                      "admission_refusals", "peak_active", "peak_committed", "queue_p50_ns",
+                                      "queue_p99_ns", "latency_p50_ns", "latency_p99_ns", "alloc_errors",
+                                      "kernel_ms")})
+        out["errors"] = g.errors()
+        log(f"[bench] serving: {out}")
+        return out
+    finally:
+        g.free_on_device()
+
+
+def blocks(seed):
+    """Block and multi-segment churn with the fragmentation figure per round."""
+    import gallatin_amd as ga
+    from gallatin_amd.workloads import BlockChurn
+
+    g = ga.Gallatin.generate_on_device(64 << 30, seed)
+    try:
+        w = BlockChurn(g, requests=1 << 16, rounds=20, seed=seed)
+        hist = w.run()
+        d = w.drain()
+        ops = sum(h["mallocs"] + h["frees"] for h in hist) + (d["frees"] if d else 0)
+        ms = sum(h["kernel_ms"] for h in hist) + (d["malloc_kernel_ms"] if d else 0.0)
+        out = {
+            "workload": "64 GiB pool, 2^16 requests of floor(2^U(16,28)) bytes over 20 rounds, 60% mallocs / "
+            "40% frees per round in one concurrent kernel",
+            "mallocs_frees_per_s": rate(ops, ms),
+            "mallocs": sum(h["mallocs"] for h in hist),
+            "misses": sum(h["misses"] for h in hist),
+            "tag_errors": sum(h["tag_errors"] for h in hist),
+            "largest_request_bytes_per_round": [h["largest_request_bytes"] for h in hist],
+            "errors": g.errors(),
+        }
+        log(f"[bench] blocks: {out}")
+        return out
+    finally:
+        g.free_on_device()
+
+
+def slab(seed):
+    """Slab hash build on an autotuned allocator, then hipGraph-replayed batches."""
+    import gallatin_amd as ga
+    from gallatin_amd import autotune
+
+    pool = 32 << 30
+    best = autotune.tune(pool, slab_keys=1 << 22, slab_buckets=1 << 16, batch_requests=1 << 22, size_classes=13,
+                         replays=3)
+    seg, mn, mx = best["triple"]
+    g = ga.Gallatin.generate_on_device(pool, seed, segment_size=seg, min_size=mn, max_size=mx)
+    try:
+        t = ga.SlabTable(g, 1 << 20, seed=seed)
+        ins = t.insert(1 << 26, key_seed=seed)
+        q = t.query(1 << 26, key_seed=seed)
+        n = 1 << 24
+        with ga.Batch(g, n) as b:
+            b.generate(n, seed=seed, size_classes=13)
+            r = b.run(ga.FORM_GRAPH, width=best["width"], run=best["run"], replays=100)
+        out = {
+            "workload": "32 GiB pool; 2^26 random u64 keys into a 2^20-bucket slab hash (120-byte slabs); then "
+            "2^24 requests of 16 B-64 KiB per hipGraph replay, 100 replays, on the live table",
+            "autotuned": {"segment_size": seg, "min_size": mn, "max_size": mx, "width": best["width"],
+                          "run": best["run"],
+                          "trials": [{"triple": list(x["triple"]), "rate": x["rate"]} for x in best["trials"]]},
+            "slab_inserts_per_s": rate(ins["keys"], ins["kernel_ms"]),
+            "slabs": ins["slabs"],
+            "max_chain": ins["max_chain"],
+            "insert_failed": ins["failed"],
+            "query_missed": q["failed"] + q["wrong"],
+            "batch_mallocs_frees_per_s": rate(r["mallocs"] + r["frees"], r["total_ms"]),
+            "batch_misses": r["misses"],
+            "batch_tag_errors": r["tag_errors"],
+        }
+        t.destroy()
+        out["errors"] = g.errors()
+        log(f"[bench] slab: {out}")
+        return out
+    finally:
+        g.free_on_device()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +220,9 @@ def main():
     ap.add_argument("--rounds", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the other workloads (forms, serving, blocks, slab)")
+    ap.add_argument("--serving-requests", type=int, default=200000, help="requests of the serving workload")
+    ap.add_argument("--serving-rate", type=float, default=50000.0)
     ap.add_argument("--cpu-threads-log2", type=int, default=22, help="oracle sample size (threads per round)")
     ap.add_argument(
         "--copy-calibration",
@@ -223,6 +374,12 @@ def main():
         "wave_hot_65536_words": ga.atomic_rate(ga.ATOMIC_WAVE_HOT, 1 << 22, 16, 65536),
     }
     log(f"[bench] atomic rates {out['atomic_rates_per_s']}")
+
+    if not args.headline_only:
+        out["forms"] = forms(args.seed)
+        out["serving"] = serving(args.seed, args.serving_requests, args.serving_rate)
+        out["blocks"] = blocks(args.seed)
+        out["slab"] = slab(args.seed)
 
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")

@@ -32,6 +32,7 @@ struct run_counters {
 constexpr int kEventPairs = 96;
 
 struct impl_ops;
+struct serve_bins;
 
 }  // namespace
 
@@ -93,6 +94,9 @@ struct gal_serve {
   unsigned long long* d_hist = nullptr;   // per-page malloc latency histogram
   std::vector<uint64_t> arrival_ns;       // host copy of the arrivals
   std::vector<uint32_t> page_sizes;       // distinct page sizes of the loaded requests
+  uint32_t* d_order = nullptr;            // request indices by bin
+  std::vector<uint32_t> bin_start, bin_count;  // per non-empty bin
+  std::vector<double> bin_work;           // pages grown and drained per bin (for dealing waves)
   uint64_t max_need = 0;                  // largest request, bytes
   bool ran = false;
 };
@@ -125,7 +129,9 @@ struct impl_ops {
   // serving loop (serve.inl)
   uint64_t (*serve_headroom)(gal_allocator* h, const std::vector<uint32_t>& page_sizes);
   unsigned (*serve_resident_blocks)();
-  void (*serve_launch)(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks);
+  void (*serve_launch)(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks,
+                       const serve_bins& bins);
+  uint32_t (*serve_bin)(uint64_t page_bytes);
 };
 
 // ---------------------------------------------------------------- kernels
@@ -626,19 +632,27 @@ struct impl {
     }
     return (unsigned)resident;
   }
-  static void serve_launch(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks) {
+  static void serve_launch(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks,
+                           const serve_bins& bins) {
     serve_state* st = (serve_state*)s->d_state;
     serve_start_kernel<<<1, 64, 0, s->h->stream>>>(st);
     serve_kernel<A><<<blocks, 256, 0, s->h->stream>>>((A*)s->h->dev, (const serve_req*)s->d_reqs, s->n, st,
                                                       s->d_checksums, s->d_admit, s->d_done, s->d_hist, headroom,
-                                                      deadline_ticks, run);
+                                                      deadline_ticks, run, bins);
+  }
+  // the bin of a page size: the class its pages claim from (big slices by
+  // slice count), block and segment pages in the last bin
+  static uint32_t serve_bin(uint64_t page_bytes) {
+    const uint32_t t = page_bytes < M ? 0u : (uint32_t)(gu::get_first_bit_bigger(page_bytes) - A::kSmallestBits);
+    return t < kServeBins - 1 ? t : kServeBins - 1;
   }
 
   static constexpr impl_ops ops() {
     return impl_ops{S,         M,          B,       &create,       &destroy,           &describe,
                     &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
                     &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, sizeof(table_t), &slab_init,
-                    &slab_launch, &segment_words, &serve_headroom, &serve_resident_blocks, &serve_launch};
+                    &slab_launch, &segment_words, &serve_headroom, &serve_resident_blocks, &serve_launch,
+                    &serve_bin};
   }
 };
 
@@ -1342,7 +1356,7 @@ int gal_serve_generate(const gal_serve_workload* w, gal_request* out) {
 int gal_serve_destroy(gal_serve* s) {
   if (!s) return GAL_ERR_INVALID;
   if (s->h) (void)hipStreamSynchronize(s->h->stream);
-  void* bufs[] = {s->d_reqs, s->d_checksums, s->d_admit, s->d_done, s->d_state, s->d_hist};
+  void* bufs[] = {s->d_reqs, s->d_checksums, s->d_admit, s->d_done, s->d_state, s->d_hist, s->d_order};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1360,6 +1374,7 @@ int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out) {
                   hipMalloc(&s->d_admit, capacity * 8) == hipSuccess &&
                   hipMalloc(&s->d_done, capacity * 8) == hipSuccess &&
                   hipMalloc(&s->d_state, sizeof(serve_state)) == hipSuccess &&
+                  hipMalloc(&s->d_order, capacity * 4) == hipSuccess &&
                   hipMalloc(&s->d_hist, kHistBuckets * sizeof(unsigned long long)) == hipSuccess;
   if (!ok) {
     gal_serve_destroy(s);
@@ -1390,6 +1405,31 @@ int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n) {
   }
   if (hipMemcpy(s->d_reqs, dev.data(), n * sizeof(serve_req), hipMemcpyHostToDevice) != hipSuccess)
     return GAL_ERR_HIP;
+  // stable counting sort by bin; empty bins dropped
+  std::vector<uint32_t> raw(n), counts(kServeBins, 0);
+  std::vector<double> work(kServeBins, 0.0);
+  for (uint64_t i = 0; i < n; i++) {
+    raw[i] = s->h->ops->serve_bin(reqs[i].page_bytes);
+    counts[raw[i]]++;
+    work[raw[i]] += 2.0 * reqs[i].pages + 2.0;
+  }
+  std::vector<uint32_t> compact(kServeBins, 0), next(kServeBins, 0);
+  s->bin_start.clear();
+  s->bin_count.clear();
+  s->bin_work.clear();
+  uint32_t pos = 0;
+  for (uint32_t b = 0; b < kServeBins; b++) {
+    if (!counts[b]) continue;
+    compact[b] = (uint32_t)s->bin_start.size();
+    next[b] = pos;
+    s->bin_start.push_back(pos);
+    s->bin_count.push_back(counts[b]);
+    s->bin_work.push_back(work[b]);
+    pos += counts[b];
+  }
+  std::vector<uint32_t> order(n);
+  for (uint64_t i = 0; i < n; i++) order[next[raw[i]]++] = (uint32_t)i;
+  if (hipMemcpy(s->d_order, order.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) return GAL_ERR_HIP;
   s->n = n;
   s->ran = false;
   return GAL_OK;
@@ -1435,6 +1475,44 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
   unsigned blocks = (unsigned)((slots + 255u) / 256u);
   if (blocks > resident) blocks = resident;
   const uint64_t deadline_ticks = (uint64_t)(deadline_s * 1e9 / h->tick_ns);
+  // deal the waves to the bins in proportion to their work, one at least
+  // each while there are enough
+  serve_bins bins;
+  std::memset(&bins, 0, sizeof(bins));
+  bins.order = s->d_order;
+  bins.nbins = (uint32_t)s->bin_start.size();
+  const uint32_t waves = blocks * 4;
+  {
+    const uint32_t nb = bins.nbins;
+    double total = 0.0;
+    for (double w : s->bin_work) total += w;
+    std::vector<uint32_t> share(nb, 0);
+    std::vector<std::pair<double, uint32_t>> frac;
+    uint32_t dealt = 0;
+    for (uint32_t b = 0; b < nb; b++) {
+      const double exact = total > 0 ? waves * s->bin_work[b] / total : 0.0;
+      share[b] = (uint32_t)exact;
+      dealt += share[b];
+      frac.push_back({exact - share[b], b});
+    }
+    std::sort(frac.begin(), frac.end(), [](auto& x, auto& y) { return x.first > y.first; });
+    for (uint32_t k = 0; dealt < waves && k < nb; k++, dealt++) share[frac[k].second]++;
+    if (waves >= nb)
+      for (uint32_t b = 0; b < nb; b++)
+        if (share[b] == 0) {
+          uint32_t big = (uint32_t)(std::max_element(share.begin(), share.end()) - share.begin());
+          share[big]--;
+          share[b]++;
+        }
+    uint32_t acc = 0;
+    for (uint32_t b = 0; b < nb; b++) {
+      bins.start[b] = s->bin_start[b];
+      bins.count[b] = s->bin_count[b];
+      bins.wave_start[b] = acc;
+      acc += share[b];
+    }
+    bins.wave_start[nb] = acc;
+  }
 
   gal_run_result base;
   std::memset(&base, 0, sizeof(base));
@@ -1443,7 +1521,7 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
         hipMemsetAsync(s->d_hist, 0, kHistBuckets * sizeof(unsigned long long), h->stream) != hipSuccess)
       return GAL_ERR_HIP;
     if (!t.begin()) return GAL_ERR_INVALID;
-    h->ops->serve_launch(s, blocks, run, headroom, deadline_ticks);
+    h->ops->serve_launch(s, blocks, run, headroom, deadline_ticks, bins);
     t.end(0);
     return GAL_OK;
   });

@@ -46,13 +46,28 @@ struct alignas(128) serve_line {
   unsigned long long v;
 };
 struct serve_state {
-  serve_line cursor;          // next ticket
+  serve_line cursors[32];     // next ticket of each request bin
   serve_line committed;       // admitted page bytes not yet allocated
   serve_line peak_committed;  // largest value `committed` reached at an admission
   // written once per wave, at its end
   unsigned long long page_mallocs, page_frees, page_retries;
   unsigned long long done, aborted, refused;
   unsigned long long t0;  // start of the loop (wall-clock ticks)
+};
+
+// Requests are binned by the class their pages claim from: each bin is a
+// queue in arrival order with a ticket counter of its own, and each wave
+// serves one bin (waves dealt to bins in proportion to their work), so the
+// growing lanes of a wave claim as one group and its draining lanes return
+// pages of the same blocks. A wave whose bin has handed out every ticket
+// moves to the next bin; once it has found every bin empty it leaves.
+constexpr uint32_t kServeBins = 32;
+struct serve_bins {
+  const uint32_t* order;            // request indices, bin after bin, each in arrival order
+  uint32_t start[kServeBins];       // first position of each bin in `order`
+  uint32_t count[kServeBins];       // requests of each bin
+  uint32_t wave_start[kServeBins + 1];  // waves [wave_start[b], wave_start[b+1]) start on bin b
+  uint32_t nbins;
 };
 
 // 16 buckets per octave; exact below 32 ticks
@@ -131,7 +146,7 @@ template <class A>
 __global__ void __launch_bounds__(256)
     serve_kernel(A* a, const serve_req* reqs, uint64_t n, serve_state* st, uint64_t* checksums,
                  uint64_t* admit_ticks, uint64_t* done_ticks, unsigned long long* hist_out, uint64_t headroom,
-                 uint64_t deadline, uint32_t run) {
+                 uint64_t deadline, uint32_t run, serve_bins bins) {
   __shared__ uint32_t hist[kHistBuckets];
   for (uint32_t i = threadIdx.x; i < kHistBuckets; i += blockDim.x) hist[i] = 0;
   __syncthreads();
@@ -143,6 +158,11 @@ __global__ void __launch_bounds__(256)
 
   uint32_t state = kSeqIdle;
   uint64_t ticket = 0;
+  // this wave's bin, and how many bins it has found out of tickets
+  const uint64_t my_wave = global_wave();
+  uint32_t bin = 0;
+  while (bin + 1 < bins.nbins && my_wave >= bins.wave_start[bin + 1]) bin++;
+  uint32_t tried = 0;
   serve_req r{0, 0, 0, 0};
   uint64_t stride = 0;          // bytes between the pages of one claim (0: one page per claim)
   uint32_t step = 0;            // pages written so far
@@ -162,21 +182,26 @@ __global__ void __launch_bounds__(256)
     const uint64_t now = wall_ticks() - t0;
     const bool late = now > deadline;
 
-    // ---- tickets for idle lanes
+    // ---- tickets for idle lanes, from this wave's bin
     const uint64_t idle = wave::ballot(state == kSeqIdle);
     if (idle) {
       const int leader = wave::first_lane(idle);
+      const uint32_t k = wave::popc(idle);
       uint64_t base = 0;
-      if ((int)lane == leader) base = atom::add(&st->cursor.v, (unsigned long long)wave::popc(idle));
+      if ((int)lane == leader) base = atom::add(&st->cursors[bin].v, (unsigned long long)k);
       base = wave::readlane64(base, leader);
       if (state == kSeqIdle) {
-        ticket = base + wave::popc(idle & wave::lanemask_lt());
-        if (ticket < n) {
+        const uint64_t t = base + wave::popc(idle & wave::lanemask_lt());
+        if (t < bins.count[bin]) {
+          ticket = bins.order[bins.start[bin] + t];
           r = reqs[ticket];
           state = kSeqWaiting;
-        } else {
-          state = kSeqExhausted;
         }
+      }
+      if (base + k >= bins.count[bin]) {  // the bin is out of tickets
+        tried++;
+        bin = bin + 1 == bins.nbins ? 0 : bin + 1;
+        if (tried >= bins.nbins && state == kSeqIdle) state = kSeqExhausted;
       }
     }
 
