@@ -400,9 +400,10 @@ class Serving:
         _capi.check("gal_serve_load", _lib.gal_serve_load(self._s, arr.ctypes.data_as(ctypes.c_void_p), len(arr)))
         self.n = len(arr)
 
-    def run(self, slots=16384, run=1, headroom=SERVE_DEFAULT_HEADROOM, deadline_s=60.0):
+    def run(self, slots=16384, run=1, headroom=SERVE_DEFAULT_HEADROOM, deadline_s=60.0, window_s=0.0):
         r = _capi.GalServeResult()
-        _capi.check("gal_serve_run", _lib.gal_serve_run(self._s, slots, run, headroom, deadline_s, ctypes.byref(r)))
+        _capi.check("gal_serve_run",
+                    _lib.gal_serve_run(self._s, slots, run, headroom, deadline_s, window_s, ctypes.byref(r)))
         return r.as_dict()
 
     def results(self):

@@ -120,6 +120,9 @@ class GalServeResult(ctypes.Structure):
         ("queue_p99_ns", ctypes.c_double),
         ("latency_p50_ns", ctypes.c_double),
         ("latency_p99_ns", ctypes.c_double),
+        ("window_pages", u64),
+        ("window_ms", ctypes.c_double),
+        ("first_dry_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -194,7 +197,10 @@ _SIGNATURES = {
     "gal_serve_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
     "gal_serve_destroy": (ctypes.c_int, [vp]),
     "gal_serve_load": (ctypes.c_int, [vp, vp, u64]),
-    "gal_serve_run": (ctypes.c_int, [vp, u32, u32, u64, ctypes.c_double, ctypes.POINTER(GalServeResult)]),
+    "gal_serve_run": (
+        ctypes.c_int,
+        [vp, u32, u32, u64, ctypes.c_double, ctypes.c_double, ctypes.POINTER(GalServeResult)],
+    ),
     "gal_serve_results": (
         ctypes.c_int,
         [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), u64],

@@ -12,11 +12,13 @@ What it chooses:
 How: every candidate triple gets a fresh allocator over the trial pool; the
 trial is the workload itself at reduced size (a slab hash build with
 `slab_keys` keys, then `replays` batched services of `batch_requests` mixed
-requests for each (width, run)); each triple keeps its best (width, run); the
-triple with the highest combined rate (slab inserts plus batched
-mallocs+frees, over the summed kernel time) wins. A `measure` function can
-stand in for the GPU trial (tests). Results can be cached in a JSON file
-keyed by the workload and the candidate set.
+requests for each (width, run) tried). The search has two stages: every
+triple is measured with one probe setting (the last width and run given),
+then the `finalists` best triples are measured over the whole (width, run)
+grid. The highest combined rate (slab inserts plus batched mallocs+frees,
+over the summed kernel time) wins. A `measure` function can stand in for the
+GPU trial (tests). Results can be cached in a JSON file keyed by the
+workload and the candidate set.
 """
 import json
 import os
@@ -76,13 +78,13 @@ def gpu_trial(triple, pool_bytes, slab_keys, slab_buckets, batch_requests, size_
 
 def tune(pool_bytes, min_size=128, segment_sizes=(4 << 20, 16 << 20, 64 << 20), slab_keys=1 << 22,
          slab_buckets=1 << 16, batch_requests=1 << 22, size_classes=13, replays=3, widths=WIDTHS, runs=RUNS,
-         measure=None, cache=None):
+         measure=None, cache=None, finalists=2):
     """Returns {"triple", "width", "run", "rate", "trials"}."""
     cands = candidates(min_size, segment_sizes)
     if not cands:
         raise ValueError(f"no legal instantiation with min_size={min_size} and segment sizes {segment_sizes}")
     key = json.dumps([pool_bytes, min_size, cands, slab_keys, slab_buckets, batch_requests, size_classes, replays,
-                      list(widths), list(runs)])
+                      list(widths), list(runs), finalists])
     if cache and os.path.exists(cache):
         with open(cache) as f:
             stored = json.load(f)
@@ -91,23 +93,31 @@ def tune(pool_bytes, min_size=128, segment_sizes=(4 << 20, 16 << 20, 64 << 20), 
             hit["triple"] = tuple(hit["triple"])
             return hit
     measure = measure or gpu_trial
-    best, trials = None, []
-    for triple in cands:
-        m = measure(triple, pool_bytes, slab_keys, slab_buckets, batch_requests, size_classes, replays, widths, runs)
+
+    def score(triple, ws, rs):
+        m = measure(triple, pool_bytes, slab_keys, slab_buckets, batch_requests, size_classes, replays, ws, rs)
         if not m["batch"] and batch_requests:
-            trials.append({"triple": triple, "rate": 0.0, "note": "every batch trial missed"})
-            continue
-        (w, r), b = max(m["batch"].items(), key=lambda kv: kv[1]["ops"] / kv[1]["ms"]) if m["batch"] else \
-            ((64, 1), {"ops": 0, "ms": 0.0})
+            return {"triple": triple, "rate": 0.0, "note": "every batch trial missed"}
+        if m["batch"]:
+            (w, r), b = max(m["batch"].items(), key=lambda kv: kv[1]["ops"] / kv[1]["ms"])
+        else:
+            (w, r), b = (ws[-1], rs[-1]), {"ops": 0, "ms": 0.0}
         ops = m["slab"]["ops"] + b["ops"]
         ms = m["slab"]["ms"] + b["ms"]
-        rate = ops / (ms * 1e-3) if ms > 0 else 0.0
-        rec = {"triple": triple, "width": w, "run": r, "rate": rate,
-               "slab_rate": m["slab"]["ops"] / (m["slab"]["ms"] * 1e-3) if m["slab"]["ms"] > 0 else None,
-               "batch_rate": b["ops"] / (b["ms"] * 1e-3) if b["ms"] > 0 else None}
-        trials.append(rec)
-        if best is None or rate > best["rate"]:
-            best = rec
+        return {"triple": triple, "width": w, "run": r, "rate": ops / (ms * 1e-3) if ms > 0 else 0.0,
+                "slab_rate": m["slab"]["ops"] / (m["slab"]["ms"] * 1e-3) if m["slab"]["ms"] > 0 else None,
+                "batch_rate": b["ops"] / (b["ms"] * 1e-3) if b["ms"] > 0 else None}
+
+    probe = [score(t, (widths[-1],), (runs[-1],)) for t in cands]
+    trials = list(probe)
+    ranked = sorted((p for p in probe if p["rate"] > 0), key=lambda p: -p["rate"])
+    best = None
+    for p in ranked[:finalists]:
+        full = score(p["triple"], widths, runs) if (len(widths), len(runs)) != (1, 1) else p
+        full["stage"] = "grid"
+        trials.append(full)
+        if best is None or full["rate"] > best["rate"]:
+            best = full
     if best is None:
         raise RuntimeError("no candidate served the trial without misses")
     out = dict(best)

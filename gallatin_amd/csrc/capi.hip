@@ -130,7 +130,7 @@ struct impl_ops {
   uint64_t (*serve_headroom)(gal_allocator* h, const std::vector<uint32_t>& page_sizes);
   unsigned (*serve_resident_blocks)();
   void (*serve_launch)(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks,
-                       const serve_bins& bins);
+                       const serve_bins& bins, uint64_t window_ticks);
   uint32_t (*serve_bin)(uint64_t page_bytes);
 };
 
@@ -633,12 +633,12 @@ struct impl {
     return (unsigned)resident;
   }
   static void serve_launch(gal_serve* s, unsigned blocks, uint32_t run, uint64_t headroom, uint64_t deadline_ticks,
-                           const serve_bins& bins) {
+                           const serve_bins& bins, uint64_t window_ticks) {
     serve_state* st = (serve_state*)s->d_state;
     serve_start_kernel<<<1, 64, 0, s->h->stream>>>(st);
     serve_kernel<A><<<blocks, 256, 0, s->h->stream>>>((A*)s->h->dev, (const serve_req*)s->d_reqs, s->n, st,
                                                       s->d_checksums, s->d_admit, s->d_done, s->d_hist, headroom,
-                                                      deadline_ticks, run, bins);
+                                                      deadline_ticks, run, bins, window_ticks);
   }
   // the bin of a page size: the class its pages claim from (big slices by
   // slice count), block and segment pages in the last bin
@@ -1461,9 +1461,10 @@ static double hist_quantile(const std::vector<unsigned long long>& hist, double 
   return 0.0;
 }
 
-int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s,
+int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s, double window_s,
                   gal_serve_result* out) {
-  if (!s || s->n == 0 || slots == 0 || run == 0 || run > kServeMaxRun || !(deadline_s > 0.0) || deadline_s > 3600.0)
+  if (!s || s->n == 0 || slots == 0 || run == 0 || run > kServeMaxRun || !(deadline_s > 0.0) || deadline_s > 3600.0 ||
+      !(window_s >= 0.0))
     return GAL_ERR_INVALID;
   gal_allocator* h = s->h;
   if (headroom == GAL_SERVE_DEFAULT_HEADROOM) headroom = h->ops->serve_headroom(h, s->page_sizes);
@@ -1475,6 +1476,7 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
   unsigned blocks = (unsigned)((slots + 255u) / 256u);
   if (blocks > resident) blocks = resident;
   const uint64_t deadline_ticks = (uint64_t)(deadline_s * 1e9 / h->tick_ns);
+  const uint64_t window_ticks = window_s > 0.0 ? (uint64_t)(window_s * 1e9 / h->tick_ns) : ~0ull;
   // deal the waves to the bins in proportion to their work, one at least
   // each while there are enough
   serve_bins bins;
@@ -1521,7 +1523,7 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
         hipMemsetAsync(s->d_hist, 0, kHistBuckets * sizeof(unsigned long long), h->stream) != hipSuccess)
       return GAL_ERR_HIP;
     if (!t.begin()) return GAL_ERR_INVALID;
-    h->ops->serve_launch(s, blocks, run, headroom, deadline_ticks, bins);
+    h->ops->serve_launch(s, blocks, run, headroom, deadline_ticks, bins, window_ticks);
     t.end(0);
     return GAL_OK;
   });
@@ -1546,6 +1548,9 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
   r.page_retries = st.page_retries;
   r.admission_refusals = st.refused;
   r.peak_committed = st.peak_committed.v;
+  r.window_pages = st.window_pages;
+  r.window_ms = window_s > 0.0 ? window_s * 1e3 : 0.0;
+  r.first_dry_ms = st.first_dry == ~0ull ? -1.0 : (double)st.first_dry * h->tick_ns * 1e-6;
   r.alloc_errors = base.alloc_errors;
   r.slots = (uint64_t)blocks * 256;
   r.run = run;

@@ -52,6 +52,8 @@ struct serve_state {
   // written once per wave, at its end
   unsigned long long page_mallocs, page_frees, page_retries;
   unsigned long long done, aborted, refused;
+  unsigned long long window_pages;  // pages written plus drained before the window's end
+  unsigned long long first_dry;     // first time a wave found every request queue empty (~0: never)
   unsigned long long t0;  // start of the loop (wall-clock ticks)
 };
 
@@ -111,16 +113,8 @@ __device__ inline void write_page(uint64_t* p, uint64_t link, uint64_t id, uint3
   }
 }
 
-// fold words 1 .. w-1 of page p into h; returns the link word
-__device__ inline uint64_t read_page(const uint64_t* p, uint32_t w, uint64_t& h) {
-  const page_pair* q = (const page_pair*)p;
-  const page_pair a = q[0];
-  page_pair b = {0, 0}, c = {0, 0}, d = {0, 0};
-  if (w > 2) b = q[1];
-  if (w > 4) {
-    c = q[2];
-    d = q[3];
-  }
+// fold words 1 .. w-1 of a header already loaded as four pairs; returns the link
+__device__ inline uint64_t fold_page(page_pair a, page_pair b, page_pair c, page_pair d, uint32_t w, uint64_t& h) {
   h = gw::fold(h, a.y);
   if (w > 2) h = gw::fold(gw::fold(h, b.x), b.y);
   if (w > 4) h = gw::fold(gw::fold(gw::fold(gw::fold(h, c.x), c.y), d.x), d.y);
@@ -128,7 +122,10 @@ __device__ inline uint64_t read_page(const uint64_t* p, uint32_t w, uint64_t& h)
 }
 
 __global__ void serve_start_kernel(serve_state* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) atom::store(&st->t0, (unsigned long long)wall_ticks());
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    atom::store(&st->first_dry, ~0ull);
+    atom::store(&st->t0, (unsigned long long)wall_ticks());
+  }
 }
 
 // Pages of one sequence are claimed `run` at a time (1..64): a growing lane
@@ -146,7 +143,7 @@ template <class A>
 __global__ void __launch_bounds__(256)
     serve_kernel(A* a, const serve_req* reqs, uint64_t n, serve_state* st, uint64_t* checksums,
                  uint64_t* admit_ticks, uint64_t* done_ticks, unsigned long long* hist_out, uint64_t headroom,
-                 uint64_t deadline, uint32_t run, serve_bins bins) {
+                 uint64_t deadline, uint32_t run, serve_bins bins, uint64_t window) {
   __shared__ uint32_t hist[kHistBuckets];
   for (uint32_t i = threadIdx.x; i < kHistBuckets; i += blockDim.x) hist[i] = 0;
   __syncthreads();
@@ -177,10 +174,36 @@ __global__ void __launch_bounds__(256)
   uint64_t unflushed = 0;       // allocated bytes not yet taken off `committed`
   bool cut = false;             // aborted by the deadline
   uint32_t nm = 0, nf = 0, nr = 0, nd = 0, na = 0, nref = 0;
+  uint32_t nw = 0;  // pages written and drained before the end of the measuring window
 
   for (uint32_t it = 0;; it++) {
     const uint64_t now = wall_ticks() - t0;
     const bool late = now > deadline;
+    const bool in_window = now <= window;
+
+    // Loads whose results are needed later in the iteration are issued
+    // first, so that their latency overlaps the ticket atomic: the header of
+    // the lane's current page (a growing lane folds its previous page, a
+    // draining lane the page it frees next) and, while the wave has lanes to
+    // admit, the admission budget.
+    uint64_t* const top_page = page;
+    const bool reading = (state == kSeqGrowing || state == kSeqDraining) && page != nullptr;
+    const uint32_t top_words = gw::page_words(r.page_bytes);
+    page_pair q0 = {0, 0}, q1 = {0, 0}, q2 = {0, 0}, q3 = {0, 0};
+    if (reading) {
+      const page_pair* q = (const page_pair*)page;
+      q0 = q[0];
+      if (top_words > 2) q1 = q[1];
+      if (top_words > 4) {
+        q2 = q[2];
+        q3 = q[3];
+      }
+    }
+    uint64_t room = 0, seen = 0;
+    if (wave::ballot(state == kSeqIdle || state == kSeqWaiting) && lane == 0) {
+      room = (uint64_t)atom::load(free_segs) * A::kBytesPerSegment;
+      seen = atom::load(&st->committed.v);
+    }
 
     // ---- tickets for idle lanes, from this wave's bin
     const uint64_t idle = wave::ballot(state == kSeqIdle);
@@ -202,6 +225,9 @@ __global__ void __launch_bounds__(256)
         tried++;
         bin = bin + 1 == bins.nbins ? 0 : bin + 1;
         if (tried >= bins.nbins && state == kSeqIdle) state = kSeqExhausted;
+        if (tried == bins.nbins && (int)lane == leader)  // this wave found every queue empty
+          (void)__hip_atomic_fetch_min(&st->first_dry, (unsigned long long)now, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
       }
     }
 
@@ -236,10 +262,10 @@ __global__ void __launch_bounds__(256)
       const uint64_t need = arrived ? (uint64_t)r.pages * r.page_bytes : 0ull;
       const uint64_t total = wave_sum64(need);
       const int leader = wave::first_lane(want);
+      room = wave::readlane64(room, 0);
+      seen = wave::readlane64(seen, 0);
       if ((int)lane == leader) {
-        // both loads in flight at once; the add only when the ask looks like it fits
-        const uint64_t room = (uint64_t)atom::load(free_segs) * A::kBytesPerSegment;
-        const uint64_t seen = atom::load(&st->committed.v);
+        // the add only when the ask looks like it fits
         // all arrived lanes, else the leader alone
         for (int pass = 0; pass < 2 && !granted; pass++) {
           const uint64_t ask = pass == 0 ? total : need;
@@ -301,9 +327,10 @@ __global__ void __launch_bounds__(256)
       const uint32_t w = gw::page_words(r.page_bytes);
       write_page(p, (uint64_t)page | (mark ? kRunStart : 0ull), r.id, step, w);
       mark = false;
-      if (page) (void)read_page(page, w, h);
+      if (page) (void)fold_page(q0, q1, q2, q3, w, h);
       page = p;
       step++;
+      if (in_window) nw++;
       if (step == r.pages) state = kSeqDraining;
     }
 
@@ -312,14 +339,15 @@ __global__ void __launch_bounds__(256)
     // the lane's pending slot; the wave frees every pending claim in one call
     // (one release fence) once kFreeBatch lanes hold one, or a lane needs its
     // slot for the next release, or the wave leaves.
-    const bool drain = state == kSeqDraining && page != nullptr;
+    const bool drain = state == kSeqDraining && page != nullptr && page == top_page && reading;
     const uint64_t draining = wave::ballot(drain);
     bool release = false;
     uint64_t* first = nullptr;
     if (draining) {
       if (drain) {
         first = page;
-        const uint64_t link = read_page(first, gw::page_words(r.page_bytes), h);
+        const uint64_t link = fold_page(q0, q1, q2, q3, top_words, h);
+        if (in_window) nw++;
         page = (uint64_t*)(link & ~kRunStart);
         drained++;
         release = (link & kRunStart) != 0;
@@ -380,7 +408,7 @@ __global__ void __launch_bounds__(256)
 
   // per-wave totals
   const uint64_t tm = wave_sum64(nm), tf = wave_sum64(nf), tr = wave_sum64(nr);
-  const uint64_t td = wave_sum64(nd), ta = wave_sum64(na), tref = wave_sum64(nref);
+  const uint64_t td = wave_sum64(nd), ta = wave_sum64(na), tref = wave_sum64(nref), tw = wave_sum64(nw);
   if (lane == 0) {
     if (tm) atom::add(&st->page_mallocs, (unsigned long long)tm);
     if (tf) atom::add(&st->page_frees, (unsigned long long)tf);
@@ -388,6 +416,7 @@ __global__ void __launch_bounds__(256)
     if (td) atom::add(&st->done, (unsigned long long)td);
     if (ta) atom::add(&st->aborted, (unsigned long long)ta);
     if (tref) atom::add(&st->refused, (unsigned long long)tref);
+    if (tw) atom::add(&st->window_pages, (unsigned long long)tw);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < kHistBuckets; i += blockDim.x)

@@ -346,6 +346,10 @@ typedef struct gal_serve_result {
   double queue_p99_ns;
   double latency_p50_ns;       /* arrival -> last page freed */
   double latency_p99_ns;
+  uint64_t window_pages;       /* pages written plus pages read back and freed before window_s */
+  double window_ms;            /* the measuring window (0: none asked) */
+  double first_dry_ms;         /* when a slot first found every request queue empty (-1: never):
+                                  a window that ends before it measures the loop saturated */
 } gal_serve_result;
 
 #define GAL_SERVE_DEFAULT_HEADROOM (~(uint64_t)0)
@@ -363,9 +367,11 @@ int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n);
  * pool when such a class has several blocks per segment (recycled blocks of
  * a segment stay hidden until a tenth of them are back, reference
  * gallatin.cuh:1646). Requests not finished deadline_s
- * seconds after the start are aborted (their pages are freed). Returns
+ * seconds after the start are aborted (their pages are freed). window_s > 0
+ * also counts the pages written and freed in the first window_s seconds
+ * (window_pages), the sustained rate while the queues are not empty. Returns
  * GAL_ERR_INVALID if a request could never be admitted. */
-int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s,
+int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s, double window_s,
                   gal_serve_result* out);
 /* per request of the last run: checksum (0 if aborted), admission and
  * completion times in ns after the start (~0 if aborted) */

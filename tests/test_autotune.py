@@ -31,14 +31,19 @@ def test_candidates_are_the_compiled_min128_triples():
     assert autotune.candidates(128, segment_sizes=(16 * MiB,)) == [(16 * MiB, 128, 4096)]
 
 
-def fake_measure(rates):
-    """Trials whose batch rate per (width, run) and slab rate come from `rates`."""
+def fake_measure(rates, calls=None):
+    """Trials whose batch rate per (width, run) and slab rate come from `rates`;
+    only the (width, run) settings asked for are measured."""
 
     def measure(triple, pool, slab_keys, slab_buckets, batch_requests, classes, replays, widths, runs):
+        if calls is not None:
+            calls.append((triple, tuple(widths), tuple(runs)))
         slab_rate, batch = rates[triple]
+        asked = {(w, r) for w in widths for r in runs}
         return {
             "slab": {"ops": slab_keys, "ms": slab_keys / slab_rate * 1e3},
-            "batch": {wr: {"ops": batch_requests * 2, "ms": batch_requests * 2 / r * 1e3} for wr, r in batch.items()},
+            "batch": {wr: {"ops": batch_requests * 2, "ms": batch_requests * 2 / r * 1e3}
+                      for wr, r in batch.items() if wr in asked},
         }
 
     return measure
@@ -46,14 +51,19 @@ def fake_measure(rates):
 
 def test_picks_best_width_run_per_triple_and_best_triple(tmp_path):
     c = autotune.candidates(128)
-    rates = {t: (1e9, {(64, 1): 1e9, (32, 16): 2e9}) for t in c}
+    # the probe setting (last width, last run) = (64, 16)
+    rates = {t: (1e9, {(64, 16): 1e9, (32, 16): 2e9}) for t in c}
     best_t = c[1]
-    rates[best_t] = (1e9, {(64, 1): 1e9, (16, 4): 4e9, (8, 1): 3e9})
-    out = autotune.tune(1 << 30, slab_keys=1000, batch_requests=1000, measure=fake_measure(rates))
+    rates[best_t] = (1e9, {(64, 16): 1.5e9, (16, 4): 4e9, (8, 1): 3e9})
+    rates[c[0]] = (1e9, {(64, 16): 0.5e9, (8, 1): 9e9})  # probes worst: never gets the grid
+    calls = []
+    out = autotune.tune(1 << 30, slab_keys=1000, batch_requests=1000, measure=fake_measure(rates, calls))
     assert out["triple"] == best_t and (out["width"], out["run"]) == (16, 4)
     # combined rate: 1000 slab ops at 1e9/s and 2000 batch ops at 4e9/s
     assert out["rate"] == pytest.approx(3000 / (1000 / 1e9 + 2000 / 4e9))
-    assert len(out["trials"]) == len(c)
+    probes = [x for x in calls if x[1] == (64,) and x[2] == (16,)]
+    grids = [x for x in calls if x not in probes]
+    assert len(probes) == len(c) and len(grids) == 2 and c[0] not in {g[0] for g in grids}
 
     cache = str(tmp_path / "tune.json")
     a = autotune.tune(1 << 30, slab_keys=1000, batch_requests=1000, measure=fake_measure(rates), cache=cache)
@@ -67,7 +77,7 @@ def test_picks_best_width_run_per_triple_and_best_triple(tmp_path):
 
 def test_a_triple_whose_trials_all_missed_is_skipped():
     c = autotune.candidates(128)
-    rates = {t: (1e9, {(64, 1): 1e9}) for t in c}
+    rates = {t: (1e9, {(64, 16): 1e9}) for t in c}
     rates[c[0]] = (1e12, {})  # every batch trial missed
     out = autotune.tune(1 << 30, slab_keys=10, batch_requests=10, measure=fake_measure(rates))
     assert out["triple"] != c[0]

@@ -67,10 +67,13 @@ def test_serving_mixed_sizes_alone_equals_mix(run):
     try:
         with ga.Serving(g, len(reqs)) as s:
             s.load(reqs)
-            r = s.run(slots=8192, run=run, deadline_s=20.0)
+            r = s.run(slots=8192, run=run, deadline_s=20.0, window_s=0.002)
             say(f"  mix {r}")
             cs, admit, done = s.results()
             check_run(r, reqs, cs)
+            # pages written plus pages freed inside the first 2 ms: some, and at most all of them
+            assert 0 < r["window_pages"] <= 2 * int(reqs["pages"].astype(np.int64).sum())
+            assert r["first_dry_ms"] > 0
             assert r["peak_active"] > 256, "sequences did not overlap"
             assert (done >= admit).all()
             # the same requests alone

@@ -69,4 +69,5 @@ def test_autotune_small_real_trial():
     for tr in out["trials"]:
         say(f"  {tr}")
     assert out["triple"] in autotune.candidates(128)
-    assert out["rate"] > 0 and len(out["trials"]) == len(autotune.candidates(128, (4 << 20, 16 << 20, 64 << 20)))
+    n_cands = len(autotune.candidates(128, (4 << 20, 16 << 20, 64 << 20)))
+    assert out["rate"] > 0 and len(out["trials"]) == n_cands + min(2, n_cands)  # probes, then the grid of 2
