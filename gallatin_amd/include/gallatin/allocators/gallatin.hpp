@@ -43,6 +43,8 @@ constexpr int kRequestBlockOomAttempts = 2; // failures with no segment anywhere
 constexpr int kBlockAttempts = 4;           // GALLATIN_MALLOC_BLOCK_ATTEMPTS (500) on OOM
 constexpr int kSegmentAttempts = 4;         // GALLATIN_MALLOC_SEGMENT_ATTEMPTS (500) on OOM
 constexpr int kPinnedWalk = 30;             // SHARED_BLOCK_COUNTER_CUTOFF
+constexpr int kDirectAttachPass = 2;        // passes a block request waits on a held class lock ...
+constexpr uint64_t kDirectAttachBlocks = 4; // ... before attaching its own segment (classes with <= 4 blocks per segment)
 constexpr uint32_t kPinnedSlotsClass0 = 128;
 // a pinned slot holds 0 (empty: a replacement ran out of memory), a block id
 // + 1, or kSlotBusy while one lane replaces or refills it
@@ -173,9 +175,17 @@ struct Gallatin {
           if (++oom >= kRequestBlockOomAttempts) return -1;
           continue;
         }
-        if (acquire_tree_lock(tree)) {
+        const bool locked = acquire_tree_lock(tree);
+        // A class with few blocks per segment needs a fresh segment for
+        // nearly every block; after a pass spent waiting for the lane that
+        // holds the class lock, a request attaches a segment of its own
+        // instead of queueing behind it (it over-attaches at most that
+        // class's few blocks). A single lane always gets the lock.
+        const bool direct = !locked && pass >= kDirectAttachPass &&
+                            table_type::blocks_in_segment(tree) <= kDirectAttachBlocks;
+        if (locked || direct) {
           int64_t got = gather_new_segment(tree);
-          release_tree_lock(tree);
+          if (locked) release_tree_lock(tree);
           if (got == -1) {
             attempts++;
             if (++oom >= kRequestBlockOomAttempts) return -1;
