@@ -523,20 +523,37 @@ struct Gallatin {
     return (uint64_t)block * kSlicesPerBlock;
   }
 
-  // contiguous segments under the segment lock (reference :790-826)
-  __device__ __noinline__ uint64_t malloc_segment_allocation(uint64_t n) {
-    for (int spin = 0; !acquire_tree_lock(kNumTrees); spin++) {
-      if (spin > (1 << 22)) return utils::kFail;
-      __builtin_amdgcn_s_sleep(1);
+  // contiguous segments under the segment lock (reference :790-826). The
+  // calling lanes (`lanes`, all active, the call uniform) find the word the
+  // claim starts at together (veb_tree::gather_plan), which reads the tree
+  // one word per lane per round trip; one lane holds the lock and claims.
+  // With one lane the result is the reference's.
+  __device__ __noinline__ uint64_t malloc_segment_allocation(uint64_t n, uint64_t lanes) {
+    const int leader = wave::first_lane(lanes);
+    uint32_t locked = 0;
+    if ((int)wave::lane_id() == leader) {
+      for (int spin = 0; spin <= (1 << 22); spin++) {
+        if (acquire_tree_lock(kNumTrees)) {
+          locked = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
     }
+    if (!wave::readlane(locked, leader)) return utils::kFail;
     atom::fence_acquire();
-    uint64_t first = segment_tree.gather_multiple(n);
-    if (first != utils::kFail) {
-      atom::sub(&control[kCtlFreeSegs], (uint32_t)n);
-      if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error(kErrSetup);
+    const uint64_t start = segment_tree.gather_plan(n, lanes);
+    uint64_t first = utils::kFail;
+    if ((int)wave::lane_id() == leader && start != utils::kFail) {
+      first = segment_tree.gather_multiple(n, start);
+      if (first != utils::kFail) {
+        atom::sub(&control[kCtlFreeSegs], (uint32_t)n);
+        if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error(kErrSetup);
+      }
     }
+    first = wave::readlane64(first, leader);
     atom::fence_release();
-    release_tree_lock(kNumTrees);
+    if ((int)wave::lane_id() == leader) release_tree_lock(kNumTrees);
     return first == utils::kFail ? utils::kFail : first * kBlocksPerSegment * kSlicesPerBlock;
   }
 
@@ -546,25 +563,31 @@ struct Gallatin {
     return table.memory + (offset / per) * bytes_per_segment + (offset % per) * table_type::tree_alloc_size(tree);
   }
 
-  // Block and segment requests, one lane of the calling lanes at a time.
-  // Kept out of line: the slice path inlined into a kernel then saves no
-  // registers for it.
+  // Block and segment requests, one request of the calling lanes at a time:
+  // a block request runs in its own lane; a segment request runs with every
+  // calling lane helping to search the segment tree. Kept out of line: the
+  // slice path inlined into a kernel then saves no registers for it.
   __device__ __noinline__ void* malloc_large(uint64_t size, uint64_t tid, int block_tree) {
     const bool is_block = (uint32_t)block_tree < kNumTrees;
     const uint64_t n = is_block ? 0 : (size - 1) / bytes_per_segment + 1;
+    const uint64_t callers = wave::active_mask();
     void* out = nullptr;
-    for (uint64_t pending = wave::active_mask(); pending; pending &= pending - 1) {
-      if ((int)wave::lane_id() == wave::first_lane(pending)) {
-        uint64_t offset = utils::kFail;
-        if (is_block) {
+    for (uint64_t pending = callers; pending; pending &= pending - 1) {
+      const int who = wave::first_lane(pending);
+      if (wave::readlane((uint32_t)is_block, who)) {
+        if ((int)wave::lane_id() == who) {
+          uint64_t offset = utils::kFail;
           for (int a = 0; a < kBlockAttempts && offset == utils::kFail; a++)
             offset = malloc_block_allocation((uint32_t)block_tree, tid);
           if (offset != utils::kFail) out = offset_to_allocation(offset, (uint32_t)block_tree);
-        } else if (n + kNumTrees + 1 < 0xFFFFull) {
-          for (int a = 0; a < kSegmentAttempts && offset == utils::kFail; a++)
-            offset = malloc_segment_allocation(n);
-          if (offset != utils::kFail) out = offset_to_allocation(offset, 0);
         }
+      } else {
+        const uint64_t runs = wave::readlane64(n, who);
+        uint64_t offset = utils::kFail;
+        if (runs + kNumTrees + 1 < 0xFFFFull)
+          for (int a = 0; a < kSegmentAttempts && offset == utils::kFail; a++)
+            offset = malloc_segment_allocation(runs, callers);
+        if ((int)wave::lane_id() == who && offset != utils::kFail) out = offset_to_allocation(offset, 0);
       }
     }
     return out;

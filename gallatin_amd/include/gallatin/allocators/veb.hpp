@@ -203,13 +203,66 @@ struct veb_tree {
     atom::fetch_or(&b[w], utils::bitmask(count));
   }
 
+  // Where the gather rule (gather_bottom) would start its successful claim
+  // of n items, reading the item words as they are now: the word of the
+  // attempt that succeeds, or kFail. The calling lanes (`lanes`, all active,
+  // the call uniform) load the words together, one word per lane per round
+  // trip, from the top down; the rule itself runs on the loaded words as
+  // uniform scalar code, with gather_bottom's control flow.
+  __device__ uint64_t gather_plan(uint64_t n, uint64_t lanes) const {
+    const uint64_t* b = bits[num_layers - 1];
+    const uint32_t m = wave::popc(lanes);
+    const uint32_t rank = wave::popc(lanes & wave::lanemask_lt());
+    int64_t hi = -1;  // the loaded words are (hi - m, hi]
+    uint64_t mine = 0;
+    auto get = [&](uint64_t w) -> uint64_t {
+      if ((int64_t)w > hi || (int64_t)w <= hi - (int64_t)m) {
+        hi = (int64_t)w;
+        const int64_t x = hi - (int64_t)rank;
+        mine = x >= 0 ? atom::load(&b[x]) : 0ull;
+      }
+      const int holder = wave::first_lane(wave::ballot(rank == (uint32_t)(hi - (int64_t)w)) & lanes);
+      return wave::readlane64(mine, holder);
+    };
+    uint64_t w = words[num_layers - 1] - 1;
+    while (w != 0) {
+      const uint64_t start = w;
+      uint64_t need = n;
+      const int run = utils::trailing_ones(get(w));
+      if (run == 0) {
+        w--;
+        continue;
+      }
+      if ((uint64_t)run >= need) return start;
+      need -= (uint64_t)run;
+      bool ok = true;
+      while (need > 64) {
+        if (w == 0) return utils::kFail;
+        w--;
+        if (get(w) != ~0ull) {
+          ok = false;
+          break;
+        }
+        need -= 64;
+      }
+      if (!ok) continue;
+      if (w == 0) return utils::kFail;
+      w--;
+      const uint64_t top = utils::bitmask(need) << (64 - need);
+      if ((get(w) & top) == top) return start;
+    }
+    return utils::kFail;
+  }
+
   // Claim n contiguous items in the item layer, searching words from the top
   // down: a run is the trailing ones of a word, extended downward through full
   // words and the high bits of the next lower word (reference veb.cuh:323-452).
-  // A run may not extend below word 0 and never starts in word 0.
-  __device__ uint64_t gather_bottom(uint64_t n) {
+  // A run may not extend below word 0 and never starts in word 0. `from`
+  // starts the search at a lower word (one that gather_plan found); attempts
+  // above it would have failed and left the words as they were.
+  __device__ uint64_t gather_bottom(uint64_t n, uint64_t from = ~0ull) {
     uint64_t* b = bits[num_layers - 1];
-    uint64_t w = words[num_layers - 1] - 1;
+    uint64_t w = from < words[num_layers - 1] ? from : words[num_layers - 1] - 1;
     while (w != 0) {
       uint64_t need = n;
       int run = utils::trailing_ones(atom::load(&b[w]));
@@ -263,8 +316,8 @@ struct veb_tree {
   }
 
   // claim a run of n items and clear the summary bits of emptied words
-  __device__ uint64_t gather_multiple(uint64_t n) {
-    uint64_t found = gather_bottom(n);
+  __device__ uint64_t gather_multiple(uint64_t n, uint64_t from = ~0ull) {
+    uint64_t found = gather_bottom(n, from);
     if (found == utils::kFail) return found;
     uint64_t start = found, count = n;
     for (int layer = num_layers - 2; layer >= 0; layer--) {
@@ -273,7 +326,11 @@ struct veb_tree {
       start = lo;
       count = hi - lo;
       for (uint64_t w = start; w < start + count; w++) {
-        if (atom::load(&bits[layer + 1][w]) == 0) atom::fetch_and(&bits[layer][w >> 6], ~(1ull << (w & 63)));
+        if (atom::load(&bits[layer + 1][w]) != 0) continue;
+        atom::fetch_and(&bits[layer][w >> 6], ~(1ull << (w & 63)));
+        // an insert into the word between the check and the clear set this
+        // very bit (it saw an empty word): put it back (as remove does)
+        if (atom::load(&bits[layer + 1][w]) != 0) atom::fetch_or(&bits[layer][w >> 6], 1ull << (w & 63));
       }
     }
     return found;
