@@ -49,9 +49,10 @@ def ops_array(ops):
 class Gallatin:
     """Host handle of one device allocator Gallatin<segment_size, min_size, max_size>."""
 
-    def __init__(self, handle, info):
+    def __init__(self, handle, info, owned=True):
         self._h = handle
         self.info = info
+        self._owned = owned
 
     # reference gallatin.cuh:324 generate_on_device(max_bytes, seed, print_info, running_calloc)
     @classmethod
@@ -73,11 +74,32 @@ class Gallatin:
             obj.print_info()
         return obj
 
+    # reference gallatin.cuh:427 generate_on_device_host: the pool is mapped host memory
+    @classmethod
+    def generate_on_device_host(
+        cls,
+        max_bytes,
+        seed,
+        print_info=False,
+        segment_size=DEFAULT_SEGMENT,
+        min_size=DEFAULT_MIN,
+        max_size=DEFAULT_MAX,
+    ):
+        h = ctypes.c_void_p()
+        _capi.check("gal_create_host",
+                    _lib.gal_create_host(segment_size, min_size, max_size, max_bytes, seed, ctypes.byref(h)))
+        info = _capi.GalInfo()
+        _capi.check("gal_get_info", _lib.gal_get_info(h, ctypes.byref(info)))
+        obj = cls(h, info)
+        if print_info:
+            obj.print_info()
+        return obj
+
     # reference gallatin.cuh:643 free_on_device
     def free_on_device(self):
-        if self._h is not None:
+        if self._h is not None and self._owned:
             _capi.check("gal_destroy", _lib.gal_destroy(self._h))
-            self._h = None
+        self._h = None
 
     def __enter__(self):
         return self
@@ -268,6 +290,71 @@ def free_global_allocator():
 
 def print_global_stats():
     _capi.check("gal_print_global_stats", _lib.gal_print_global_stats())
+
+
+def allocated_size(size, segment_size=DEFAULT_SEGMENT, min_size=DEFAULT_MIN, max_size=DEFAULT_MAX):
+    """Bytes malloc grants a request of `size` (gal_allocated_size; host only)."""
+    out = ctypes.c_uint64()
+    _capi.check("gal_allocated_size",
+                _lib.gal_allocated_size(segment_size, min_size, max_size, size, ctypes.byref(out)))
+    return out.value
+
+
+# reference global_allocator.cuh:98-236: a global allocator over host memory, and both
+def init_global_allocator_host(num_bytes, seed, print_info=True):
+    _capi.check("gal_init_global_allocator_host",
+                _lib.gal_init_global_allocator_host(num_bytes, seed, int(bool(print_info))))
+
+
+def free_global_allocator_host():
+    _capi.check("gal_free_global_allocator_host", _lib.gal_free_global_allocator_host())
+
+
+def print_global_stats_host():
+    _capi.check("gal_print_global_stats_host", _lib.gal_print_global_stats_host())
+
+
+def init_global_allocator_combined(num_bytes, host_bytes, seed, print_info=True):
+    _capi.check("gal_init_global_allocator_combined",
+                _lib.gal_init_global_allocator_combined(num_bytes, host_bytes, seed, int(bool(print_info))))
+
+
+def free_global_allocator_combined():
+    _capi.check("gal_free_global_allocator_combined", _lib.gal_free_global_allocator_combined())
+
+
+def print_global_stats_combined():
+    _capi.check("gal_print_global_stats_combined", _lib.gal_print_global_stats_combined())
+
+
+def global_handles():
+    """(device, host) global allocators as non-owning Gallatin views (None if absent)."""
+    d, h = ctypes.c_void_p(), ctypes.c_void_p()
+    _capi.check("gal_global_handles", _lib.gal_global_handles(ctypes.byref(d), ctypes.byref(h)))
+    out = []
+    for x in (d, h):
+        if not x.value:
+            out.append(None)
+            continue
+        info = _capi.GalInfo()
+        _capi.check("gal_get_info", _lib.gal_get_info(x, ctypes.byref(info)))
+        out.append(Gallatin(x, info, owned=False))
+    return tuple(out)
+
+
+def global_fused(n_threads, size):
+    """global_malloc_fused / global_free_fused from n threads: (on_device, on_host, misses, tag_errors)."""
+    a, b, c, d = (ctypes.c_uint64() for _ in range(4))
+    _capi.check("gal_global_fused", _lib.gal_global_fused(n_threads, size, ctypes.byref(a), ctypes.byref(b),
+                                                          ctypes.byref(c), ctypes.byref(d)))
+    return a.value, b.value, c.value, d.value
+
+
+def global_poison(size):
+    """Guard-check results of the reference's poison sequence for one size (gal_global_poison)."""
+    codes = (ctypes.c_int32 * 7)()
+    _capi.check("gal_global_poison", _lib.gal_global_poison(size, codes, 7))
+    return list(codes)
 
 
 def global_smoke(n_threads, size):

@@ -161,6 +161,20 @@ _SIGNATURES = {
     "gal_free_global_allocator": (ctypes.c_int, []),
     "gal_print_global_stats": (ctypes.c_int, []),
     "gal_global_smoke": (ctypes.c_int, [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "gal_create_host": (ctypes.c_int, [u64, u64, u64, u64, u64, ctypes.POINTER(vp)]),
+    "gal_allocated_size": (ctypes.c_int, [u64, u64, u64, u64, ctypes.POINTER(u64)]),
+    "gal_init_global_allocator_host": (ctypes.c_int, [u64, u64, ctypes.c_int]),
+    "gal_free_global_allocator_host": (ctypes.c_int, []),
+    "gal_print_global_stats_host": (ctypes.c_int, []),
+    "gal_init_global_allocator_combined": (ctypes.c_int, [u64, u64, u64, ctypes.c_int]),
+    "gal_free_global_allocator_combined": (ctypes.c_int, []),
+    "gal_print_global_stats_combined": (ctypes.c_int, []),
+    "gal_global_handles": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+    "gal_global_fused": (
+        ctypes.c_int,
+        [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)],
+    ),
+    "gal_global_poison": (ctypes.c_int, [u64, ctypes.POINTER(ctypes.c_int32), u32]),
     "gal_run_plain": (ctypes.c_int, [vp, u64, u64, u64, u32, ctypes.POINTER(GalRunResult)]),
     "gal_mixed_reset": (ctypes.c_int, [vp, u64, u32, u64]),
     "gal_mixed_round": (ctypes.c_int, [vp, u32, u32, ctypes.POINTER(GalRunResult)]),

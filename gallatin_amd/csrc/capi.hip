@@ -106,6 +106,8 @@ namespace {
 struct impl_ops {
   uint64_t seg, min, max;
   void* (*create)(uint64_t pool, uint64_t seed);
+  void* (*create_host)(uint64_t pool, uint64_t seed);
+  uint64_t (*allocated_size)(uint64_t size);
   void (*destroy)(void* dev);
   int (*describe)(gal_allocator* h);
   int (*snapshot)(gal_allocator* h, uint8_t* out);
@@ -403,6 +405,67 @@ bool gather_would_succeed(const uint64_t* b, uint64_t words, uint64_t n) {
   return false;
 }
 
+// device pool first, host pool when it has none (global_malloc_fused); counts
+// where each allocation landed
+__global__ void global_fused_malloc_kernel(uint64_t n, uint64_t size, uint64_t** ptrs, run_counters* c,
+                                           unsigned long long* on_host) {
+  const uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool got = false, bad = false, miss = false, host = false;
+  if (i < n) {
+    uint64_t* p = (uint64_t*)ga::global_malloc_fused(size);
+    ptrs[i] = p;
+    if (p) {
+      got = true;
+      host = !ga::global_gallatin->owns_allocation(p);
+      bad = !tag_in(p, gw::payload_tag(11, i), GAL_VERIFY_ATOMIC);
+    } else {
+      miss = true;
+    }
+  }
+  count(c, got, false, miss, bad);
+  const uint64_t h = wave::ballot(host);
+  if (h && (int)wave::lane_id() == wave::first_lane(h)) atom::add(on_host, (unsigned long long)wave::popc(h));
+}
+__global__ void global_fused_free_kernel(uint64_t n, uint64_t** ptrs, run_counters* c) {
+  const uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  bool freed = false, bad = false;
+  if (i < n && ptrs[i]) {
+    uint64_t* p = ptrs[i];
+    bad = !tag_out(p, gw::payload_tag(11, i), GAL_VERIFY_ATOMIC);
+    ga::global_free_fused(p);
+    freed = true;
+  }
+  count(c, false, freed, false, bad);
+}
+
+// The reference's poison sequence (tests/src/poison_test.cu:60-128) for one
+// size, in one lane: an intact allocation, then guard words and the byte
+// after the caller's bytes overwritten. codes[k] = the check's result.
+__device__ inline char* poison_alloc_and_write(uint64_t size) {
+  char* p = (char*)ga::global_malloc_poison(size);
+  if (p)
+    for (uint64_t i = 0; i < size; i++) p[i] = (char)((i * i + 1) % 256);
+  return p;
+}
+__global__ void global_poison_kernel(uint64_t size, int* codes) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int k = 0; k < 7; k++) {
+    char* p = poison_alloc_and_write(size);
+    if (!p) {
+      codes[k] = -1;
+      continue;
+    }
+    uint64_t* w = (uint64_t*)p;
+    if (k == 1) w[-2] = 17;  // the header's size word
+    if (k == 2) w[-1] = 17;  // the header's extra word
+    if (k == 3) p[size] = 'c';  // the byte after the caller's bytes
+    if (k == 4) w[-1] = w[-2] = 17;
+    if (k == 5) w[-1] = w[-2] = 48;
+    if (k == 6) p[size - 1] = 'c';  // the caller's own last byte: allowed
+    codes[k] = ga::global_free_poison(p);
+  }
+}
+
 inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
 #include "batch.inl"
@@ -415,6 +478,10 @@ struct impl {
   using A = ga::Gallatin<S, M, B>;
 
   static void* create(uint64_t pool, uint64_t seed) { return (void*)A::generate_on_device(pool, seed, false); }
+  static void* create_host(uint64_t pool, uint64_t seed) {
+    return (void*)A::generate_on_device_host(pool, seed, false);
+  }
+  static uint64_t allocated_size(uint64_t size) { return A::allocated_size(size); }
   static void destroy(void* dev) { A::free_on_device((A*)dev); }
 
   static int describe(gal_allocator* h) {
@@ -475,15 +542,15 @@ struct impl {
     uint64_t pad = ((nseg * 2 + 7) / 8) * 8 - nseg * 2;
     std::memset(p, 0, pad);
     p += pad;
-    // {malloc, free} per block in the reference's packing (class << 20 | count)
-    std::vector<uint64_t> rec(nblocks * 2);
-    if (hipMemcpy2D(rec.data(), 16, host.table.blocks, sizeof(ga::Block), 16, nblocks, hipMemcpyDeviceToHost) !=
-        hipSuccess)
+    // {malloc, free} per block in the reference's packing (class << 20 | count);
+    // the padded records are copied whole (one contiguous copy) and packed here
+    std::vector<ga::Block> rec(nblocks);
+    if (hipMemcpy(rec.data(), host.table.blocks, nblocks * sizeof(ga::Block), hipMemcpyDeviceToHost) != hipSuccess)
       return GAL_ERR_HIP;
     for (uint64_t k = 0; k < nblocks; k++) {
-      const uint64_t m = rec[2 * k];
+      const uint64_t m = rec[k].malloc_counter;
       const uint32_t count = ga::counter_count(m) > 0xFFFFFu ? 0xFFFFFu : ga::counter_count(m);
-      const uint32_t pair[2] = {(ga::counter_class(m) << ga::kSnapshotClassShift) | count, (uint32_t)rec[2 * k + 1]};
+      const uint32_t pair[2] = {(ga::counter_class(m) << ga::kSnapshotClassShift) | count, rec[k].free_counter};
       std::memcpy(p, pair, 8);
       p += 8;
     }
@@ -648,7 +715,8 @@ struct impl {
   }
 
   static constexpr impl_ops ops() {
-    return impl_ops{S,         M,          B,       &create,       &destroy,           &describe,
+    return impl_ops{S,         M,          B,       &create,       &create_host,       &allocated_size,    &destroy,
+                    &describe,
                     &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
                     &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, sizeof(table_t), &slab_init,
                     &slab_launch, &segment_words, &serve_headroom, &serve_resident_blocks, &serve_launch,
@@ -745,8 +813,9 @@ int timed_run(gal_allocator* h, gal_run_result* out, F&& body) {
   return GAL_OK;
 }
 
-// state of the global allocator made by gal_init_global_allocator
+// state of the global allocators made by gal_init_global_allocator(_host)
 gal_allocator* g_global = nullptr;
+gal_allocator* g_global_host = nullptr;
 
 }  // namespace
 
@@ -1656,6 +1725,33 @@ int gal_atomic_rate(uint32_t mode, uint64_t threads, uint32_t iters, uint32_t ad
   return rc;
 }
 
+int gal_allocated_size(uint64_t segment_size, uint64_t min_size, uint64_t max_size, uint64_t size, uint64_t* bytes) {
+  if (!bytes || size == 0) return GAL_ERR_INVALID;
+  const impl_ops* ops = find_impl(segment_size, min_size, max_size);
+  if (!ops) return GAL_ERR_UNSUPPORTED;
+  *bytes = ops->allocated_size(size);
+  return GAL_OK;
+}
+
+int gal_create_host(uint64_t segment_size, uint64_t min_size, uint64_t max_size, uint64_t pool_bytes, uint64_t seed,
+                    gal_allocator** out) {
+  if (!out) return GAL_ERR_INVALID;
+  *out = nullptr;
+  const impl_ops* ops = find_impl(segment_size, min_size, max_size);
+  if (!ops) return GAL_ERR_UNSUPPORTED;
+  if (pool_bytes < segment_size) return GAL_ERR_INVALID;
+  void* dev = ops->create_host(pool_bytes, seed);
+  if (!dev) return GAL_ERR_OOM;
+  gal_allocator* h = nullptr;
+  int rc = make_handle(ops, dev, &h);
+  if (rc != GAL_OK) {
+    ops->destroy(dev);
+    return rc;
+  }
+  *out = h;
+  return GAL_OK;
+}
+
 int gal_init_global_allocator(uint64_t num_bytes, uint64_t seed, int print_info) {
   if (g_global) return GAL_ERR_INVALID;
   if (!ga::init_global_allocator(num_bytes, seed, print_info != 0)) return GAL_ERR_OOM;
@@ -1684,6 +1780,115 @@ int gal_print_global_stats(void) {
   ga::print_global_stats();
   fflush(stdout);
   return GAL_OK;
+}
+
+int gal_init_global_allocator_host(uint64_t num_bytes, uint64_t seed, int print_info) {
+  if (g_global_host) return GAL_ERR_INVALID;
+  if (!ga::init_global_allocator_host(num_bytes, seed, print_info != 0)) return GAL_ERR_OOM;
+  gal_allocator* h = nullptr;
+  int rc = make_handle(find_impl(16 * MiB, 16, 4096), (void*)ga::get_global_allocator_host(), &h);
+  if (rc != GAL_OK) {
+    ga::free_global_allocator_host();
+    return rc;
+  }
+  g_global_host = h;
+  return GAL_OK;
+}
+
+int gal_free_global_allocator_host(void) {
+  if (!g_global_host) return GAL_ERR_INVALID;
+  ga::free_global_allocator_host();
+  g_global_host->dev = nullptr;
+  gal_destroy(g_global_host);
+  g_global_host = nullptr;
+  return GAL_OK;
+}
+
+int gal_print_global_stats_host(void) {
+  if (!g_global_host) return GAL_ERR_INVALID;
+  ga::print_global_stats_host();
+  fflush(stdout);
+  return GAL_OK;
+}
+
+int gal_init_global_allocator_combined(uint64_t num_bytes, uint64_t host_bytes, uint64_t seed, int print_info) {
+  int rc = gal_init_global_allocator(num_bytes, seed, print_info);
+  if (rc != GAL_OK) return rc;
+  rc = gal_init_global_allocator_host(host_bytes, seed, print_info);
+  if (rc != GAL_OK) gal_free_global_allocator();
+  return rc;
+}
+
+int gal_free_global_allocator_combined(void) {
+  const int a = gal_free_global_allocator();
+  const int b = gal_free_global_allocator_host();
+  return a != GAL_OK ? a : b;
+}
+
+int gal_print_global_stats_combined(void) {
+  if (!g_global || !g_global_host) return GAL_ERR_INVALID;
+  ga::print_global_stats_combined();
+  fflush(stdout);
+  return GAL_OK;
+}
+
+int gal_global_handles(gal_allocator** device, gal_allocator** host) {
+  if (device) *device = g_global;
+  if (host) *host = g_global_host;
+  return GAL_OK;
+}
+
+int gal_global_fused(uint64_t n_threads, uint64_t size, uint64_t* on_device, uint64_t* on_host, uint64_t* misses,
+                     uint64_t* tag_errors) {
+  if (!g_global || !g_global_host || n_threads == 0) return GAL_ERR_INVALID;
+  uint64_t** ptrs = nullptr;
+  unsigned long long* host_count = nullptr;
+  if (hipMalloc(&ptrs, n_threads * 8) != hipSuccess || hipMalloc(&host_count, 8) != hipSuccess ||
+      hipMemset(host_count, 0, 8) != hipSuccess) {
+    if (ptrs) (void)hipFree(ptrs);
+    if (host_count) (void)hipFree(host_count);
+    return GAL_ERR_OOM;
+  }
+  gal_run_result a, b;
+  int rc = timed_run(g_global, &a, [&](launch_timer& t) {
+    if (!t.begin()) return GAL_ERR_INVALID;
+    global_fused_malloc_kernel<<<blocks_for(n_threads, 256), 256, 0, g_global->stream>>>(n_threads, size, ptrs,
+                                                                                        g_global->d_counters,
+                                                                                        host_count);
+    t.end(0);
+    return GAL_OK;
+  });
+  if (rc == GAL_OK)
+    rc = timed_run(g_global, &b, [&](launch_timer& t) {
+      if (!t.begin()) return GAL_ERR_INVALID;
+      global_fused_free_kernel<<<blocks_for(n_threads, 256), 256, 0, g_global->stream>>>(n_threads, ptrs,
+                                                                                        g_global->d_counters);
+      t.end(1);
+      return GAL_OK;
+    });
+  unsigned long long hc = 0;
+  if (rc == GAL_OK && hipMemcpy(&hc, host_count, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = GAL_ERR_HIP;
+  (void)hipFree(ptrs);
+  (void)hipFree(host_count);
+  if (rc != GAL_OK) return rc;
+  if (on_device) *on_device = a.mallocs - hc;
+  if (on_host) *on_host = hc;
+  if (misses) *misses = a.misses;
+  if (tag_errors) *tag_errors = a.tag_errors + b.tag_errors + (b.frees == a.mallocs ? 0 : 1);
+  return GAL_OK;
+}
+
+int gal_global_poison(uint64_t size, int32_t* codes, uint32_t cap) {
+  if (!g_global || !codes || cap < 7 || size == 0) return GAL_ERR_INVALID;
+  int* d = nullptr;
+  if (hipMalloc(&d, 7 * sizeof(int)) != hipSuccess) return GAL_ERR_OOM;
+  global_poison_kernel<<<1, 64, 0, g_global->stream>>>(size, d);
+  int rc = hipStreamSynchronize(g_global->stream) == hipSuccess &&
+                   hipMemcpy(codes, d, 7 * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess
+               ? GAL_OK
+               : GAL_ERR_HIP;
+  (void)hipFree(d);
+  return rc;
 }
 
 int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors) {

@@ -57,6 +57,13 @@ constexpr uint32_t kPinnedSlotsMin = 4;     // MIN_PINNED_CUTOFF
 // has a lane of that class.
 constexpr uint32_t kSlotsUnderLoad = 32;
 
+// Where the pool lives (reference memory_table.cuh:61). Metadata always lives
+// in device memory; only the pool moves. host_only: pinned host memory mapped
+// into the GPU's address space (hipHostMalloc, mapped); allocations are then
+// reached over the host link. managed is not provided (HIP managed memory
+// needs XNACK to migrate pages).
+enum Gallatin_memory_type { device_only, host_only, managed };
+
 __host__ __device__ constexpr int ilog2(uint64_t x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 __host__ __device__ constexpr bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 
@@ -104,6 +111,7 @@ struct Gallatin {
   uint64_t seed;
   // device allocations, for free_on_device
   void* owned[6];
+  uint32_t memory_kind;  // Gallatin_memory_type of the pool
 
 #if defined(__HIPCC__)
   // ------------------------------------------------------------ size classes
@@ -769,9 +777,31 @@ struct Gallatin {
     return total;
   }
 
+  // Bytes a request of `size` is granted (the path malloc takes for it):
+  // a slice of its class, 2^k top-class slices, a whole block, or whole
+  // segments (reference gallatin.cuh:1152-1183, which reports the next
+  // power of two for every request below a segment).
+  __host__ __device__ static uint64_t allocated_size(uint64_t size) {
+    if (size < smallest) size = smallest;
+    const int bits = utils::get_first_bit_bigger(size);
+    const int tree = bits - kSmallestBits;
+    if (tree < (int)kNumTrees) return smallest << tree;
+    if (bits - kSmallestTreeBits < 0) return (uint64_t)biggest << (tree - (int)(kNumTrees - 1));
+    if (bits - kSmallestTreeBits < (int)kNumTrees) return 1ull << bits;
+    return ((size - 1) / bytes_per_segment + 1) * bytes_per_segment;
+  }
+
   // reference gallatin.cuh:324-421. Returns a device pointer, nullptr on failure.
   __host__ static my_type* generate_on_device(uint64_t max_bytes, uint64_t seed, bool print_info = true,
-                                              bool running_calloc = false);
+                                              bool running_calloc = false) {
+    return generate(max_bytes, seed, print_info, device_only);
+  }
+  // the same over a pool of mapped host memory (reference gallatin.cuh:427-630)
+  __host__ static my_type* generate_on_device_host(uint64_t max_bytes, uint64_t seed, bool print_info = true,
+                                                   bool running_calloc = false) {
+    return generate(max_bytes, seed, print_info, host_only);
+  }
+  __host__ static my_type* generate(uint64_t max_bytes, uint64_t seed, bool print_info, Gallatin_memory_type kind);
   __host__ static void free_on_device(my_type* dev_version);
   // called on the device pointer, as in the reference (:1855)
   __host__ void print_info();
@@ -818,9 +848,10 @@ __host__ inline unsigned grid_for(uint64_t n) {
 }  // namespace detail
 
 template <uint64_t S, uint64_t M, uint64_t B>
-__host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_bytes, uint64_t seed, bool print_info,
-                                                         bool running_calloc) {
-  (void)running_calloc;  // calloc needs device-side launches; not supported
+__host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint64_t seed, bool print_info,
+                                                       Gallatin_memory_type kind) {
+  // calloc (running_calloc) needs device-side launches; not supported
+  if (kind != device_only && kind != host_only) return nullptr;
   const uint64_t nseg = max_bytes / S;
   if (nseg == 0) return nullptr;
   const int layers = veb_tree::layers_for(nseg);
@@ -836,7 +867,8 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_b
 
   void* veb = nullptr;
   void* table_mem = nullptr;
-  void* pool = nullptr;
+  void* pool = nullptr;       // device address of the pool
+  void* pool_host = nullptr;  // host address of a host_only pool
   void* dev = nullptr;
   const size_t veb_bytes = (size_t)tree_words * 8 * (1 + kNumTrees);
   // chunk ids | blocks | queues | segment counters | slots | control
@@ -850,13 +882,20 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_b
   auto fail = [&]() -> my_type* {
     if (veb) (void)hipFree(veb);
     if (table_mem) (void)hipFree(table_mem);
-    if (pool) (void)hipFree(pool);
+    if (pool_host) (void)hipHostFree(pool_host);
+    else if (pool) (void)hipFree(pool);
     if (dev) (void)hipFree(dev);
     return nullptr;
   };
   if (hipMalloc(&veb, veb_bytes) != hipSuccess) return fail();
   if (hipMalloc(&table_mem, table_bytes) != hipSuccess) return fail();
-  if (hipMalloc(&pool, nseg * S) != hipSuccess) return fail();
+  if (kind == host_only) {
+    if (hipHostMalloc(&pool_host, nseg * S, hipHostMallocMapped) != hipSuccess) return fail();
+    std::memset(pool_host, 0, nseg * S);
+    if (hipHostGetDevicePointer(&pool, pool_host, 0) != hipSuccess) return fail();
+  } else if (hipMalloc(&pool, nseg * S) != hipSuccess) {
+    return fail();
+  }
   if (hipMalloc(&dev, sizeof(my_type)) != hipSuccess) return fail();
 
   // vEB trees: segment tree full, class trees empty
@@ -900,12 +939,13 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate_on_device(uint64_t max_b
   gallatin_fill_u32<<<detail::grid_for(nseg), 256>>>(host.table.chunk_ids, nseg, kFreeChunk);
   gallatin_init_blocks<<<detail::grid_for(nblocks), 256>>>(host.table.blocks, nblocks);
   gallatin_init_segments<<<detail::grid_for(nseg), 256>>>(host.table.segs, nseg);
-  if (hipMemset(pool, 0, nseg * S) != hipSuccess) return fail();
+  if (kind == device_only && hipMemset(pool, 0, nseg * S) != hipSuccess) return fail();
 
   host.owned[0] = veb;
   host.owned[1] = table_mem;
-  host.owned[2] = pool;
+  host.owned[2] = kind == host_only ? pool_host : pool;
   host.owned[3] = dev;
+  host.memory_kind = (uint32_t)kind;
   if (hipMemcpy(dev, &host, sizeof(my_type), hipMemcpyHostToDevice) != hipSuccess) return fail();
   gallatin_boot_kernel<my_type><<<1, 1>>>((my_type*)dev);
   if (hipDeviceSynchronize() != hipSuccess) return fail();
@@ -925,8 +965,14 @@ __host__ void Gallatin<S, M, B>::free_on_device(my_type* dev_version) {
   my_type host;
   if (!host_copy(dev_version, &host)) return;
   (void)hipDeviceSynchronize();
-  for (int i = 0; i < 3; i++)
+  for (int i = 0; i < 2; i++)
     if (host.owned[i]) (void)hipFree(host.owned[i]);
+  if (host.owned[2]) {
+    if (host.memory_kind == host_only)
+      (void)hipHostFree(host.owned[2]);
+    else
+      (void)hipFree(host.owned[2]);
+  }
   (void)hipFree(dev_version);
 }
 

@@ -105,6 +105,18 @@ int gal_supported(uint64_t* out, int cap);
 
 int gal_create(uint64_t segment_size, uint64_t min_size, uint64_t max_size,
                uint64_t pool_bytes, uint64_t seed, gal_allocator** out);
+/* Bytes malloc grants a request of `size` on a (segment, min, max) allocator:
+ * a slice, 2^k top-class slices, a whole block or whole segments (host
+ * only, no GPU). Replaces Gallatin::get_allocated_size, reference
+ * gallatin.cuh:1152-1183, which rounds every request below a segment up to
+ * a power of two. */
+int gal_allocated_size(uint64_t segment_size, uint64_t min_size, uint64_t max_size, uint64_t size, uint64_t* bytes);
+
+/* The same over a pool of pinned host memory mapped into the GPU (metadata
+ * stays in device memory). Replaces generate_on_device_host, reference
+ * gallatin.cuh:427-630 (memory_table.cuh:340-366, host_only). */
+int gal_create_host(uint64_t segment_size, uint64_t min_size, uint64_t max_size,
+                    uint64_t pool_bytes, uint64_t seed, gal_allocator** out);
 int gal_destroy(gal_allocator* h);
 int gal_get_info(gal_allocator* h, gal_info_t* info);
 int gal_print_stats(gal_allocator* h);
@@ -143,6 +155,34 @@ int gal_free_global_allocator(void);
 int gal_print_global_stats(void);
 /* one kernel of n threads: global_malloc(size), tag, verify, global_free */
 int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors);
+
+/* Global allocator over mapped host memory, and both together. Replace
+ * init_global_allocator_host / free_global_allocator_host /
+ * print_global_stats_host and the _combined forms, reference
+ * global_allocator.cuh:98-236. */
+int gal_init_global_allocator_host(uint64_t num_bytes, uint64_t seed, int print_info);
+int gal_free_global_allocator_host(void);
+int gal_print_global_stats_host(void);
+int gal_init_global_allocator_combined(uint64_t num_bytes, uint64_t host_bytes, uint64_t seed, int print_info);
+int gal_free_global_allocator_combined(void);
+int gal_print_global_stats_combined(void);
+/* the handles of the global allocators (owned by the library: not to be
+ * destroyed), for snapshots and counters; NULL when not initialised */
+int gal_global_handles(gal_allocator** device, gal_allocator** host);
+/* n threads each global_malloc_fused(size) (device pool first, then the
+ * host pool), tag; a second kernel checks and global_free_fused. Counts
+ * where the allocations landed. Reference global_allocator.cuh:199-216. */
+int gal_global_fused(uint64_t n_threads, uint64_t size, uint64_t* on_device, uint64_t* on_host, uint64_t* misses,
+                     uint64_t* tag_errors);
+/* The reference's poison sequence (tests/src/poison_test.cu:60-128) for one
+ * size on the device global allocator, one lane: 7 allocations by
+ * global_malloc_poison, each written and freed by global_free_poison after
+ * (1) nothing, (2) the header's size word, (3) its extra word, (4) the byte
+ * after the caller's bytes, (5) both header words = 17, (6) both = 48 were
+ * overwritten, (7) the caller's own last byte was written. codes[k] = the
+ * guard check's result (0 intact; 1-5 see global_check_poison; -1 malloc
+ * failed). */
+int gal_global_poison(uint64_t size, int32_t* codes, uint32_t cap);
 
 /* ---- workloads ----
  * Kernel durations come from HIP events recorded on the library's own stream
