@@ -117,6 +117,21 @@ class Gallatin:
     def print_info(self):
         _capi.check("gal_print_stats", _lib.gal_print_stats(self._h))
 
+    def fill(self):
+        """print_info's figures as numbers (gal_fill): per class lists of length num_trees."""
+        r = _capi.GalFillReport()
+        _capi.check("gal_fill", _lib.gal_fill(self._h, ctypes.byref(r)))
+        t = r.num_trees
+        return {
+            "free_segments": r.free_segments,
+            "max_segments": r.max_segments,
+            "listed_segments": list(r.listed_segments[:t]),
+            "live_segments": list(r.live_segments[:t]),
+            "blocks_available": list(r.blocks_available[:t]),
+            "blocks_in_use": list(r.blocks_in_use[:t]),
+            "overhead_bytes": r.overhead_bytes,
+        }
+
     @property
     def handle(self):
         return self._h

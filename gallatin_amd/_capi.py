@@ -144,6 +144,19 @@ class GalSlabResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GalFillReport(ctypes.Structure):
+    _fields_ = [
+        ("num_trees", u64),
+        ("free_segments", u64),
+        ("max_segments", u64),
+        ("listed_segments", u64 * 32),
+        ("live_segments", u64 * 32),
+        ("blocks_available", u64 * 32),
+        ("blocks_in_use", u64 * 32),
+        ("overhead_bytes", u64),
+    ]
+
+
 GAL_SERVE_DEFAULT_HEADROOM = (1 << 64) - 1
 
 _SIGNATURES = {
@@ -152,6 +165,7 @@ _SIGNATURES = {
     "gal_destroy": (ctypes.c_int, [vp]),
     "gal_get_info": (ctypes.c_int, [vp, ctypes.POINTER(GalInfo)]),
     "gal_print_stats": (ctypes.c_int, [vp]),
+    "gal_fill": (ctypes.c_int, [vp, ctypes.POINTER(GalFillReport)]),
     "gal_snapshot": (ctypes.c_int, [vp, vp, u64, ctypes.POINTER(u64)]),
     "gal_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64)]),
     "gal_errors": (ctypes.c_int, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),

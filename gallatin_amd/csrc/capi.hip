@@ -118,6 +118,7 @@ struct impl_ops {
   void (*launch_mixed_free)(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
                             bool all);
   void (*print)(gal_allocator* h);
+  int (*fill)(gal_allocator* h, gal_fill_report* out);
   // batches (batch.inl)
   void (*batch_bin)(gal_batch* b);
   void (*batch_pass)(gal_batch* b, uint32_t mode, uint32_t width_log2, uint32_t run, uint32_t verify);
@@ -586,6 +587,41 @@ struct impl {
                            0, h->stream>>>((A*)h->dev, h->d_live, n_live, round, seed, verify, all, h->d_counters);
   }
   static void print(gal_allocator* h) { ((A*)h->dev)->print_info(); }
+  static int fill(gal_allocator* h, gal_fill_report* out) {
+    A host;
+    if (!A::host_copy((const A*)h->dev, &host)) return GAL_ERR_HIP;
+    std::memset(out, 0, sizeof(*out));
+    out->num_trees = A::kNumTrees;
+    auto items = [&](const ga::veb_tree& t, uint64_t& n) {
+      const int b = t.num_layers - 1;
+      std::vector<uint64_t> w(t.words[b]);
+      if (hipMemcpy(w.data(), t.bits[b], w.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return false;
+      n = 0;
+      for (uint64_t x : w) n += (uint64_t)__builtin_popcountll(x);
+      return true;
+    };
+    if (!items(host.segment_tree, out->free_segments)) return GAL_ERR_HIP;
+    out->max_segments = host.table.num_segments;
+    for (uint32_t c = 0; c < A::kNumTrees; c++)
+      if (!items(host.sub_trees[c], out->listed_segments[c])) return GAL_ERR_HIP;
+    const uint64_t nseg = host.table.num_segments;
+    std::vector<uint32_t> ids(nseg);
+    std::vector<ga::segment_counters> segs(nseg);
+    if (hipMemcpy(ids.data(), host.table.chunk_ids, nseg * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(segs.data(), host.table.segs, nseg * sizeof(ga::segment_counters), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+      return GAL_ERR_HIP;
+    for (uint64_t x = 0; x < nseg; x++) {
+      const uint32_t c = ids[x];
+      if (c >= A::kNumTrees || segs[x].active == -1) continue;
+      const int64_t nb = (int64_t)A::table_type::blocks_in_segment(c);
+      out->live_segments[c]++;
+      out->blocks_available[c] += (uint64_t)segs[x].active;
+      out->blocks_in_use[c] += (uint64_t)(nb - segs[x].active);
+    }
+    out->overhead_bytes = h->info.metadata_bytes;
+    return GAL_OK;
+  }
   static void launch_churn(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t seed, uint64_t* d_results) {
     churn_kernel<A><<<blocks_for(n, 256), 256, 0, h->stream>>>((A*)h->dev, d_ops, n, seed, d_results, h->d_counters);
   }
@@ -717,7 +753,7 @@ struct impl {
   static constexpr impl_ops ops() {
     return impl_ops{S,         M,          B,       &create,       &create_host,       &allocated_size,    &destroy,
                     &describe,
-                    &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print,
+                    &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print, &fill,
                     &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, sizeof(table_t), &slab_init,
                     &slab_launch, &segment_words, &serve_headroom, &serve_resident_blocks, &serve_launch,
                     &serve_bin};
@@ -898,6 +934,12 @@ int gal_print_stats(gal_allocator* h) {
   h->ops->print(h);
   fflush(stdout);
   return GAL_OK;
+}
+
+int gal_fill(gal_allocator* h, gal_fill_report* out) {
+  if (!h || !out) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  return h->ops->fill(h, out);
 }
 
 int gal_snapshot(gal_allocator* h, void* buf, uint64_t cap, uint64_t* needed) {

@@ -791,6 +791,15 @@ struct Gallatin {
     return ((size - 1) / bytes_per_segment + 1) * bytes_per_segment;
   }
 
+  // the reference's names for the same figures (gallatin.cuh:1152, 1901-1910)
+  __host__ __device__ static uint64_t get_allocated_size(uint64_t size) { return allocated_size(size); }
+  __host__ __device__ static uint64_t get_blocks_per_segment(uint16_t tree) {
+    return table_type::blocks_in_segment(tree);
+  }
+  __host__ __device__ static uint64_t get_max_allocations_per_segment() {
+    return table_type::max_allocations_per_segment;
+  }
+
   // reference gallatin.cuh:324-421. Returns a device pointer, nullptr on failure.
   __host__ static my_type* generate_on_device(uint64_t max_bytes, uint64_t seed, bool print_info = true,
                                               bool running_calloc = false) {

@@ -120,6 +120,24 @@ int gal_create_host(uint64_t segment_size, uint64_t min_size, uint64_t max_size,
 int gal_destroy(gal_allocator* h);
 int gal_get_info(gal_allocator* h, gal_info_t* info);
 int gal_print_stats(gal_allocator* h);
+
+/* The figures print_info prints, as numbers (reference gallatin.cuh:1855-2011
+ * print_info / print_guided_fill / calculate_overhead, veb.cuh report_fill
+ * and report_max). Per class c < num_trees: segments its tree lists,
+ * segments formatted for it, and over those, blocks still grantable (the
+ * segments' active counts, "available") and blocks granted and not back ("in
+ * use"). */
+typedef struct gal_fill_report {
+  uint64_t num_trees;
+  uint64_t free_segments;  /* items of the segment tree (report_fill) */
+  uint64_t max_segments;   /* its universe (report_max) */
+  uint64_t listed_segments[32];
+  uint64_t live_segments[32];
+  uint64_t blocks_available[32];
+  uint64_t blocks_in_use[32];
+  uint64_t overhead_bytes; /* device memory held besides the pool */
+} gal_fill_report;
+int gal_fill(gal_allocator* h, gal_fill_report* out);
 int gal_snapshot(gal_allocator* h, void* buf, uint64_t cap, uint64_t* needed);
 
 /* Single-lane deterministic replay; results[i] per op (see above). */
