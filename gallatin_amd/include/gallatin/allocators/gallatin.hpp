@@ -318,11 +318,16 @@ struct Gallatin {
     return fresh >= 0;
   }
 
-  // nothing a block request could find: no listed segment for the class and
-  // no free segment (with one lane, exactly when request_new_block_from_tree
-  // fails)
-  __device__ bool class_out_of_memory(uint32_t tree) const {
-    return sub_trees[tree].looks_empty() && segment_tree.nothing_to_claim();
+  // nothing a block request seeded with `tid` could find: its hashed search
+  // of the class's segments fails and no free segment is left (with one
+  // lane, exactly when request_new_block_from_tree fails, which changes no
+  // state). The search, not an emptiness test, decides: a class whose only
+  // listed segment is segment 0 is invisible to hashed starts above 0
+  // (reference veb.cuh:960-975), and a stale summary bit can make an empty
+  // tree look occupied; either way a refill could only fail, and every wave
+  // of the class would queue for it.
+  __device__ bool class_out_of_memory(uint32_t tree, uint64_t tid) const {
+    return sub_trees[tree].find_random_valid_index(tid) == utils::kFail && segment_tree.nothing_to_claim();
   }
 
   // the pinned slot a group uses: start at hint % n, step past empty slots
@@ -465,7 +470,7 @@ struct Gallatin {
       if ((int)me == wave::first_lane(todo)) {
         const uint64_t lead_tid = tid;
         if (empty) {
-          refilled = !class_out_of_memory(tree) && refill_slot(tree, index, lead_tid);
+          refilled = !class_out_of_memory(tree, lead_tid) && refill_slot(tree, index, lead_tid);
         } else {
           if (replace) replace_block(tree, index, block, lead_tid);
           if (completed) recycle_block(block);  // the write-off finished the block
