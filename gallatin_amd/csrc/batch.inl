@@ -236,10 +236,11 @@ __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes,
   const uint32_t tm = wave_inclusive_sum(nm), tf = wave_inclusive_sum(nf);
   const uint32_t tx = wave_inclusive_sum(nx), tb = wave_inclusive_sum(nb);
   if (lane == 63) {
-    if (tm) atom::add(&c->mallocs, (unsigned long long)tm);
-    if (tf) atom::add(&c->frees, (unsigned long long)tf);
-    if (tx) atom::add(&c->misses, (unsigned long long)tx);
-    if (tb) atom::add(&c->tag_errors, (unsigned long long)tb);
+    counter_stripe* k = stripe_of(c);
+    if (tm) atom::add(&k->mallocs, (unsigned long long)tm);
+    if (tf) atom::add(&k->frees, (unsigned long long)tf);
+    if (tx) atom::add(&k->misses, (unsigned long long)tx);
+    if (tb) atom::add(&k->tag_errors, (unsigned long long)tb);
   }
 }
 

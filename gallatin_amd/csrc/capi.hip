@@ -25,8 +25,17 @@ namespace atom = gallatin::atom;
 
 namespace {
 
-struct run_counters {
+// Activity counters of a run, striped over cache lines: a wave adds to the
+// stripe of its index, so that the adds of a large grid do not all queue on
+// one line (same-address atomics on one line run at about 1e8/s, which a
+// kernel of 2^16 waves adding twice each would spend 1.3 ms on); the host
+// sums the stripes.
+constexpr int kCounterStripes = 64;
+struct alignas(128) counter_stripe {
   unsigned long long mallocs, frees, misses, tag_errors;
+};
+struct run_counters {
+  counter_stripe s[kCounterStripes];
 };
 
 constexpr int kEventPairs = 96;
@@ -150,12 +159,19 @@ __device__ inline bool tag_out(uint64_t* p, uint64_t tag, uint32_t verify) {
   return ok;
 }
 
+// the calling wave's stripe of the run counters
+__device__ inline counter_stripe* stripe_of(run_counters* c) {
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  return &c->s[wave % kCounterStripes];
+}
+
 // wave-aggregated counter update: one atomic per wave and counter
-__device__ inline void count(run_counters* c, bool m, bool f, bool miss, bool bad) {
+__device__ inline void count(run_counters* rc, bool m, bool f, bool miss, bool bad) {
   const uint64_t active = wave::active_mask();
   const uint32_t nm = wave::popc(wave::ballot(m)), nf = wave::popc(wave::ballot(f));
   const uint32_t nx = wave::popc(wave::ballot(miss)), nb = wave::popc(wave::ballot(bad));
   if ((int)wave::lane_id() == wave::first_lane(active)) {
+    counter_stripe* c = stripe_of(rc);
     if (nm) atom::add(&c->mallocs, (unsigned long long)nm);
     if (nf) atom::add(&c->frees, (unsigned long long)nf);
     if (nx) atom::add(&c->misses, (unsigned long long)nx);
@@ -314,8 +330,8 @@ __global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
     }
   }
   if (pending > 0) flush(pending);
-  if (lane == 0 && freed) atom::add(&c->frees, (unsigned long long)freed);
-  if (lane == 0 && bad) atom::add(&c->tag_errors, (unsigned long long)bad);
+  if (lane == 0 && freed) atom::add(&stripe_of(c)->frees, (unsigned long long)freed);
+  if (lane == 0 && bad) atom::add(&stripe_of(c)->tag_errors, (unsigned long long)bad);
 }
 
 __global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) {
@@ -840,10 +856,12 @@ int timed_run(gal_allocator* h, gal_run_result* out, F&& body) {
   run_counters c;
   if (hipMemcpy(&c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
   if (read_errors(h, &err1, nullptr) != GAL_OK) return GAL_ERR_HIP;
-  r.mallocs = c.mallocs;
-  r.frees = c.frees;
-  r.misses = c.misses;
-  r.tag_errors = c.tag_errors;
+  for (const counter_stripe& k : c.s) {
+    r.mallocs += k.mallocs;
+    r.frees += k.frees;
+    r.misses += k.misses;
+    r.tag_errors += k.tag_errors;
+  }
   r.alloc_errors = err1 - err0;
   if (out) *out = r;
   return GAL_OK;
