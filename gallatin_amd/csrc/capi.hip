@@ -731,8 +731,10 @@ struct impl {
     bool hidden = false;
     for (uint32_t c = 0; c < A::kNumTrees; c++) {
       if (!used[c]) continue;
-      if (slots_used[c])
-        bytes += (uint64_t)ga::slot_region_of(c) * A::table_type::tree_alloc_size(c) * ga::kSlicesPerBlock;
+      if (slots_used[c]) {
+        const uint64_t block = A::table_type::tree_alloc_size(c) * ga::kSlicesPerBlock;
+        bytes += (uint64_t)ga::usable_slots_of(c, h->info.num_segments * S, block) * block;
+      }
       bytes += 2 * S;
       hidden |= A::table_type::blocks_in_segment(c) > 1;
     }

@@ -51,11 +51,17 @@ constexpr uint32_t kPinnedSlotsClass0 = 128;
 constexpr uint32_t kSlotBusy = 0xFFFFFFFFu;
 constexpr uint32_t kPinnedSlotsMin = 4;     // MIN_PINNED_CUTOFF
 // Classes with fewer than kSlotsUnderLoad pinned slots get overflow slots up
-// to that count. A group uses them only when every pinned slot of its class
-// is busy being replaced, which a single lane never sees; under load they
-// let replacements of one class overlap instead of stalling every wave that
-// has a lane of that class.
-constexpr uint32_t kSlotsUnderLoad = 32;
+// to that count, which a single lane never uses. Under load (more than one
+// lane of a wave looking) a group starts at one of its class's usable slots,
+// pinned or overflow, and a group that finds every pinned slot busy looks at
+// the overflow ones: the claims of a class then spread over many block
+// counters instead of queueing on a few (same-address atomics on one line
+// run at about 1e8/s, and every wave has a lane of every class in a mixed
+// workload), and replacements of one class overlap. A class's usable slots
+// hold at most 1/kSlotShare of the pool in blocks (at least its pinned slots),
+// so that a small pool is not tied up in partly used blocks.
+constexpr uint32_t kSlotsUnderLoad = 64;
+constexpr uint64_t kSlotShare = 16;
 
 // Where the pool lives (reference memory_table.cuh:61). Metadata always lives
 // in device memory; only the pool moves. host_only: pinned host memory mapped
@@ -77,6 +83,13 @@ __host__ __device__ constexpr uint32_t overflow_slots_of(uint32_t c) {
 }
 // slot region of class c: its pinned slots, then its overflow slots
 __host__ __device__ constexpr uint32_t slot_region_of(uint32_t c) { return pinned_slots_of(c) + overflow_slots_of(c); }
+// slots of class c usable under load, for a pool of pool_bytes whose class-c
+// blocks are block_bytes each
+__host__ __device__ constexpr uint32_t usable_slots_of(uint32_t c, uint64_t pool_bytes, uint64_t block_bytes) {
+  const uint64_t budget = pool_bytes / kSlotShare / block_bytes;
+  const uint64_t lo = pinned_slots_of(c), hi = slot_region_of(c);
+  return (uint32_t)(budget < lo ? lo : budget > hi ? hi : budget);
+}
 
 template <uint64_t bytes_per_segment, uint64_t smallest, uint64_t biggest>
 struct Gallatin {
@@ -107,6 +120,7 @@ struct Gallatin {
   table_type table;
   uint32_t* slots;                // pinned blocks, class-major: block id + 1
   uint32_t slot_offset[kNumTrees];
+  uint32_t usable_slots[kNumTrees];  // usable_slots_of each class for this pool
   uint32_t* control;              // control_word array
   uint64_t seed;
   // device allocations, for free_on_device
@@ -377,15 +391,33 @@ struct Gallatin {
 
     uint32_t index, entry;
     const uint32_t lead_hint = wave::shfl(hint, leader);
-    pinned_slot(tree, lead_hint, index, entry);
+    const uint32_t usable = usable_slots[tree];
+    // Under load a group's first pass starts at slot hint % usable; an empty
+    // slot there is filled unless the class is out of memory. Otherwise (and
+    // always for a single lane) the reference's walk over the pinned slots.
+    bool placed = false;
+    if (attempt == 0 && usable > pinned_slots_of(tree) && (looking & (looking - 1)) != 0) {
+      const uint32_t si = lead_hint % usable;
+      const uint32_t se = atom::load(slot_ptr(tree, si));
+      if (se != 0u && se != kSlotBusy) {
+        index = si;
+        entry = se;
+        placed = true;
+      } else if (se == 0u && !class_out_of_memory(tree, tid)) {
+        index = si;
+        entry = 0u;
+        placed = true;
+      }
+    }
+    if (!placed) pinned_slot(tree, lead_hint, index, entry);
     index = wave::shfl(index, leader);
     entry = wave::shfl(entry, leader);
     // Every pinned slot of the class busy: the group's lanes each look at
     // one overflow slot, from a start that moves with the hint and the pass,
     // and take the first holding a block, else the first empty one (which
     // the group then fills), else stay busy.
-    if (wave::ballot(entry == kSlotBusy && overflow_slots_of(tree) > 0) & looking) {
-      const uint32_t k = overflow_slots_of(tree);
+    if (wave::ballot(entry == kSlotBusy && usable > pinned_slots_of(tree)) & looking) {
+      const uint32_t k = usable - pinned_slots_of(tree);
       const bool want = entry == kSlotBusy && k > 0;
       const uint32_t rank = wave::popc(group & wave::lanemask_lt());
       const bool look = want && rank < k;
@@ -945,6 +977,7 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint
   for (uint32_t c = 0; c < kNumTrees; c++) {
     host.slot_offset[c] = off;
     off += slot_region_of(c);
+    host.usable_slots[c] = usable_slots_of(c, nseg * S, table_type::tree_alloc_size(c) * kSlicesPerBlock);
   }
   if (hipMemset(table_mem, 0, table_bytes) != hipSuccess) return fail();
   if (nseg > 0xFFFFFFFFull) return fail();

@@ -819,11 +819,11 @@ struct Alloc {
     return true;
   }
 
-  // The GPU gives a class with fewer than 32 pinned slots overflow slots up
-  // to 32, used only when all its pinned slots are busy being replaced by
-  // other lanes; with one lane they stay empty. The snapshot lists each
-  // class's pinned slots followed by its (empty) overflow slots.
-  static uint64_t overflow_slots(uint64_t pinned) { return pinned < 32 ? 32 - pinned : 0; }
+  // The GPU gives a class with fewer than 64 pinned slots overflow slots up
+  // to 64, used only when more than one lane of a wave claims at once; with
+  // one lane they stay empty. The snapshot lists each class's pinned slots
+  // followed by its (empty) overflow slots.
+  static uint64_t overflow_slots(uint64_t pinned) { return pinned < 64 ? 64 - pinned : 0; }
 
   uint64_t total_slots() const {
     uint64_t t = 0;
