@@ -289,13 +289,23 @@ __global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
   const uint64_t base = wave_index * 64 * kFreeScanIters;
   uint32_t pending = 0;  // wave-uniform count of packed entries
   uint64_t freed = 0, bad = 0;
+  // The tags were stored by the malloc kernel, which has ended: in store
+  // mode the tag is only read here, nothing is stored to the freed memory
+  // and free_released applies. Atomic mode clears the tag (the next owner's
+  // exchange expects 0), so its frees release.
   auto flush = [&](uint32_t k) {
     bool b = false;
     if (lane < k) {
       uint64_t* p = (uint64_t*)pend_p[w][lane];
       const uint64_t j = pend_j[w][lane];
-      b = !tag_out(p, gw::payload_tag(seed, j), verify);
-      a->free(p);
+      const uint64_t tag = gw::payload_tag(seed, j);
+      if (verify == GAL_VERIFY_ATOMIC) {
+        b = !tag_out(p, tag, verify);
+        a->free(p);
+      } else {
+        b = *(volatile uint64_t*)p != tag;
+        a->free_released(p);
+      }
       live[j] = 0;
     }
     freed += k;

@@ -709,8 +709,9 @@ struct Gallatin {
   // wave (its stores are performed before any of its slices can be handed
   // out again), then one atomic instruction in which each group's first lane
   // adds the group's size to its block's free counter. A block whose count
-  // completes is recycled, one at a time.
-  __device__ __forceinline__ void free_offset(uint64_t offset, uint32_t allocs = 1u) {
+  // completes is recycled, one at a time. `release` false skips the fence,
+  // for callers whose stores to the freed memory are already released.
+  __device__ __forceinline__ void free_offset(uint64_t offset, uint32_t allocs = 1u, bool release = true) {
     const uint64_t block = offset / kSlicesPerBlock;
     const uint64_t active = wave::active_mask();
     const uint64_t group = wave::match_groups(active, block);
@@ -722,7 +723,7 @@ struct Gallatin {
       uint32_t unused;
       wave::group_scan(group, allocs, 7, unused, n);
     }
-    atom::fence_release();
+    if (release) atom::fence_release();
     bool complete = false;
     if (is_leader) {
       const uint32_t old = atom::add(&table.blocks[block].free_counter, n);
@@ -752,13 +753,27 @@ struct Gallatin {
       atom::add(&control[kCtlBadFrees], 1u);
       return;
     }
-    const uint64_t offset = (byte - segment * bytes_per_segment) / table_type::tree_alloc_size(tree) +
+    const uint64_t offset = ((byte - segment * bytes_per_segment) >> (kSmallestBits + tree)) +
                             segment * table_type::max_allocations_per_segment;
     free_offset(offset, k);
   }
 
-  // reference gallatin.cuh:1378-1461
-  __device__ __forceinline__ void free(void* allocation) {
+  // reference gallatin.cuh:1378-1461. The 8 XCD L2 caches of MI355X are not
+  // coherent with each other: a free first writes back the calling wave's
+  // stores (an agent-scope release, one L2 writeback per call), so that a
+  // dirty line of the freed memory cannot be written back over the data of
+  // its next owner on another XCD.
+  __device__ __forceinline__ void free(void* allocation) { free_impl(allocation, true); }
+
+  // free() for memory whose stores are already released: nothing the calling
+  // wave stored to it since its last agent-scope release fence (or since its
+  // kernel began: a kernel boundary releases everything) is still pending.
+  // Typical caller: a kernel that frees what an earlier kernel allocated and
+  // wrote, reading it at most. Skips the L2 writeback of free(); has no
+  // counterpart in the reference, whose L2 is coherent.
+  __device__ __forceinline__ void free_released(void* allocation) { free_impl(allocation, false); }
+
+  __device__ __forceinline__ void free_impl(void* allocation, bool release) {
     const uint64_t byte = (uint64_t)((char*)allocation - table.memory);
     const uint64_t segment = byte / bytes_per_segment;
     if (byte >= table.num_segments * bytes_per_segment) {
@@ -774,10 +789,9 @@ struct Gallatin {
       free_segment_run(segment, tree);
       return;
     }
-    const uint64_t offset =
-        (byte - segment * bytes_per_segment) / table_type::tree_alloc_size(tree) +
-        segment * table_type::max_allocations_per_segment;
-    free_offset(offset);
+    const uint64_t offset = ((byte - segment * bytes_per_segment) >> (kSmallestBits + tree)) +
+                            segment * table_type::max_allocations_per_segment;
+    free_offset(offset, 1u, release);
   }
 
   __device__ bool owns_allocation(void* allocation) const {
