@@ -151,19 +151,23 @@ def test_plain_kernel_sizes(size):
         g.free_on_device()
 
 
-def test_mixed_rounds_invariants():
+@pytest.mark.parametrize("verify", [ga.GAL_VERIFY_ATOMIC, ga.GAL_VERIFY_STORE])
+def test_mixed_rounds_invariants(verify):
+    """Atomic mode: tags exchanged in and out (sharing detected), frees with
+    free(). Store mode: the free kernel reads the tag and frees with
+    free_released (the tags were stored by the previous kernel)."""
     pool, threads, rounds = 2 << 30, 1 << 18, 6
-    say(f"[mixed] pool={pool} threads={threads} rounds={rounds}")
+    say(f"[mixed] pool={pool} threads={threads} rounds={rounds} verify={verify}")
     g = ga.Gallatin.generate_on_device(pool, 21)
     try:
         g.mixed_reset(threads, rounds, 77)
         for r in range(rounds):
-            res = g.mixed_round(r, verify=ga.GAL_VERIFY_ATOMIC)
+            res = g.mixed_round(r, verify=verify)
             offs, sizes, slots = g.mixed_live()
             say(f"  round {r}: {res} live {len(offs)} sites {g.error_sites()}")
             invariants.check_live(offs, sizes, pool, 16 * MiB, 16, 4096, oracle.route)
             assert res["tag_errors"] == 0 and res["alloc_errors"] == 0, (res, g.error_sites())
-        res = g.mixed_drain(verify=ga.GAL_VERIFY_ATOMIC)
+        res = g.mixed_drain(verify=verify)
         assert res["tag_errors"] == 0 and res["alloc_errors"] == 0
         invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
         lat = g.latency_ns()
