@@ -544,7 +544,7 @@ struct impl {
     const uint64_t nseg = host.table.num_segments, nblocks = nseg * A::kBlocksPerSegment;
     uint8_t* p = out;
     uint32_t locks = 0;
-    if (hipMemcpy(&locks, host.control + ga::kCtlLocks, 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    if (hipMemcpy(&locks, ga::ctl_word(host.control, ga::kCtlLocks), 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
     uint64_t header[8] = {nseg, A::kBlocksPerSegment, A::kNumTrees, (uint64_t)host.segment_tree.num_layers,
                           A::total_slots(), locks, 0, 0};
     std::memcpy(p, header, sizeof(header));
@@ -842,10 +842,10 @@ struct launch_timer {
 };
 
 int read_errors(gal_allocator* h, uint64_t* internal, uint64_t* bad) {
-  uint32_t ctl[ga::kCtlWords];
+  uint32_t ctl[ga::kCtlWords * ga::kCtlStride];
   if (hipMemcpy(ctl, h->control, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
-  if (internal) *internal = ctl[ga::kCtlErrors];
-  if (bad) *bad = ctl[ga::kCtlBadFrees];
+  if (internal) *internal = *ga::ctl_word(ctl, ga::kCtlErrors);
+  if (bad) *bad = *ga::ctl_word(ctl, ga::kCtlBadFrees);
   return GAL_OK;
 }
 
@@ -988,19 +988,19 @@ int gal_errors(gal_allocator* h, uint64_t* internal_errors, uint64_t* bad_frees)
 
 int gal_error_sites(gal_allocator* h, uint32_t* counts, uint32_t cap) {
   if (!h || !counts) return GAL_ERR_INVALID;
-  uint32_t ctl[ga::kCtlWords];
+  uint32_t ctl[ga::kCtlWords * ga::kCtlStride];
   if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
   if (hipMemcpy(ctl, h->control, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
-  for (uint32_t i = 0; i < cap && i < ga::kErrSites; i++) counts[i] = ctl[ga::kCtlSiteBase + i];
+  for (uint32_t i = 0; i < cap && i < ga::kErrSites; i++) counts[i] = *ga::ctl_word(ctl, ga::kCtlSiteBase + i);
   return GAL_OK;
 }
 
 int gal_stats(gal_allocator* h, uint32_t* counts, uint32_t cap) {
   if (!h || !counts) return GAL_ERR_INVALID;
-  uint32_t ctl[ga::kCtlWords];
+  uint32_t ctl[ga::kCtlWords * ga::kCtlStride];
   if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
   if (hipMemcpy(ctl, h->control, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
-  for (uint32_t i = 0; i < cap && i < ga::kStatSites; i++) counts[i] = ctl[ga::kCtlStatBase + i];
+  for (uint32_t i = 0; i < cap && i < ga::kStatSites; i++) counts[i] = *ga::ctl_word(ctl, ga::kCtlStatBase + i);
   return GAL_OK;
 }
 
@@ -1457,7 +1457,7 @@ int gal_free_bytes(gal_allocator* h, uint64_t* bytes) {
   if (!h || !bytes) return GAL_ERR_INVALID;
   uint32_t segs = 0;
   if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
-  if (hipMemcpy(&segs, h->control + ga::kCtlFreeSegs, 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  if (hipMemcpy(&segs, ga::ctl_word(h->control, ga::kCtlFreeSegs), 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
   *bytes = (uint64_t)segs * h->info.segment_size;
   return GAL_OK;
 }

@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t lane = wave::lane_id();
   const uint64_t t0 = atom::load(&st->t0);
   const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
-  uint32_t* free_segs = &a->control[ga::kCtlFreeSegs];
+  uint32_t* free_segs = ga::ctl_word(a->control, ga::kCtlFreeSegs);
 
   uint32_t state = kSeqIdle;
   uint64_t ticket = 0;

@@ -135,7 +135,7 @@ struct Gallatin {
     return (uint32_t)(uint16_t)(utils::get_first_bit_bigger(size) - kSmallestBits);
   }
 
-  __device__ uint32_t* lock_word() const { return &control[kCtlLocks]; }
+  __device__ uint32_t* lock_word() const { return ctl_word(control, kCtlLocks); }
   __device__ bool acquire_tree_lock(uint32_t tree) {
     uint32_t m = 1u << tree;
     return (atom::fetch_or(lock_word(), m) & m) == 0;
@@ -150,7 +150,7 @@ struct Gallatin {
   __device__ __noinline__ int64_t gather_new_segment(uint32_t tree) {
     uint64_t segment = segment_tree.malloc_first();
     if (segment == utils::kFail) return -1;
-    atom::sub(&control[kCtlFreeSegs], 1u);
+    atom::sub(ctl_word(control, kCtlFreeSegs), 1u);
     if (!table.setup_segment(segment, tree)) {
       count_error(kErrSetup);
       return -1;
@@ -258,7 +258,7 @@ struct Gallatin {
       if (!table.reset_tree_id(segment, tree)) count_error(kErrChunkReset);
       atom::fence();
       segment_tree.insert_force_update(segment);
-      atom::add(&control[kCtlFreeSegs], 1u);
+      atom::add(ctl_word(control, kCtlFreeSegs), 1u);
     }
   }
 
@@ -592,7 +592,7 @@ struct Gallatin {
     if ((int)wave::lane_id() == leader && start != utils::kFail) {
       first = segment_tree.gather_multiple(n, start);
       if (first != utils::kFail) {
-        atom::sub(&control[kCtlFreeSegs], (uint32_t)n);
+        atom::sub(ctl_word(control, kCtlFreeSegs), (uint32_t)n);
         if (!table.set_tree_id(first, (uint32_t)(kNumTrees + 1 + n))) count_error(kErrSetup);
       }
     }
@@ -740,7 +740,7 @@ struct Gallatin {
     if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
     atom::fence_release();
     segment_tree.return_multiple(segment, tree - kNumTrees - 1);
-    atom::add(&control[kCtlFreeSegs], tree - kNumTrees - 1);
+    atom::add(ctl_word(control, kCtlFreeSegs), tree - kNumTrees - 1);
   }
 
   // Free all k allocations of a run from malloc_run_at at once: one share
@@ -750,7 +750,7 @@ struct Gallatin {
     const uint64_t segment = byte / bytes_per_segment;
     const uint32_t tree = byte < table.num_segments * bytes_per_segment ? table.read_tree_id(segment) : kFreeChunk;
     if (tree >= kNumTrees || k == 0 || k > 64) {
-      atom::add(&control[kCtlBadFrees], 1u);
+      atom::add(ctl_word(control, kCtlBadFrees), 1u);
       return;
     }
     const uint64_t offset = ((byte - segment * bytes_per_segment) >> (kSmallestBits + tree)) +
@@ -777,12 +777,12 @@ struct Gallatin {
     const uint64_t byte = (uint64_t)((char*)allocation - table.memory);
     const uint64_t segment = byte / bytes_per_segment;
     if (byte >= table.num_segments * bytes_per_segment) {
-      atom::add(&control[kCtlBadFrees], 1u);
+      atom::add(ctl_word(control, kCtlBadFrees), 1u);
       return;
     }
     const uint32_t tree = table.read_tree_id(segment);
     if (tree == kFreeChunk || tree == kNumTrees || tree == kNumTrees + 1) {
-      atom::add(&control[kCtlBadFrees], 1u);
+      atom::add(ctl_word(control, kCtlBadFrees), 1u);
       return;
     }
     if (tree > kNumTrees) {
@@ -895,7 +895,7 @@ template <class A>
 __global__ void gallatin_boot_kernel(A* alloc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     bool ok = alloc->boot_pinned_blocks();
-    atom::store(&alloc->control[kCtlBootStatus], ok ? 1u : 2u);
+    atom::store(ctl_word(alloc->control, kCtlBootStatus), ok ? 1u : 2u);
   }
 }
 #endif
@@ -937,7 +937,7 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint
   const size_t off_segs = off_queues + ((nblocks * 4 + 255) / 256) * 256;
   const size_t off_slots = off_segs + nseg * sizeof(segment_counters);
   const size_t off_ctl = off_slots + ((total_slots() * 4 + 255) / 256) * 256;
-  const size_t table_bytes = off_ctl + kCtlWords * 4;
+  const size_t table_bytes = off_ctl + kCtlWords * kCtlStride * 4;
 
   auto fail = [&]() -> my_type* {
     if (veb) (void)hipFree(veb);
@@ -996,7 +996,7 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint
   if (hipMemset(table_mem, 0, table_bytes) != hipSuccess) return fail();
   if (nseg > 0xFFFFFFFFull) return fail();
   const uint32_t free_segs = (uint32_t)nseg;
-  if (hipMemcpy(host.control + kCtlFreeSegs, &free_segs, 4, hipMemcpyHostToDevice) != hipSuccess) return fail();
+  if (hipMemcpy(ctl_word(host.control, kCtlFreeSegs), &free_segs, 4, hipMemcpyHostToDevice) != hipSuccess) return fail();
   gallatin_fill_u32<<<detail::grid_for(nseg), 256>>>(host.table.chunk_ids, nseg, kFreeChunk);
   gallatin_init_blocks<<<detail::grid_for(nblocks), 256>>>(host.table.blocks, nblocks);
   gallatin_init_segments<<<detail::grid_for(nseg), 256>>>(host.table.segs, nseg);
@@ -1011,7 +1011,7 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint
   gallatin_boot_kernel<my_type><<<1, 1>>>((my_type*)dev);
   if (hipDeviceSynchronize() != hipSuccess) return fail();
   uint32_t status = 0;
-  if (hipMemcpy(&status, host.control + kCtlBootStatus, 4, hipMemcpyDeviceToHost) != hipSuccess || status != 1u)
+  if (hipMemcpy(&status, ctl_word(host.control, kCtlBootStatus), 4, hipMemcpyDeviceToHost) != hipSuccess || status != 1u)
     return fail();
   if (print_info)
     printf("Booted Gallatin with %u trees in range %llu-%llu and %f GB of memory %llu segments\n", kNumTrees,
@@ -1078,9 +1078,9 @@ __host__ void Gallatin<S, M, B>::print_info() {
            (unsigned long long)count, count ? 1.0 * (double)avail / (double)count : 0.0, (long long)nb,
            count ? 1.0 * (double)in_use / (double)count : 0.0, (long long)nb);
   }
-  uint32_t ctl[kCtlWords];
+  uint32_t ctl[kCtlWords * kCtlStride];
   if (hipMemcpy(ctl, host.control, sizeof(ctl), hipMemcpyDeviceToHost) == hipSuccess)
-    printf("Allocator errors %u, rejected frees %u\n", ctl[kCtlErrors], ctl[kCtlBadFrees]);
+    printf("Allocator errors %u, rejected frees %u\n", *ctl_word(ctl, kCtlErrors), *ctl_word(ctl, kCtlBadFrees));
   const uint64_t overhead = sizeof(my_type) + (uint64_t)veb_tree::layers_for(nseg) * 0 +
                             nseg * (4 + sizeof(segment_counters)) + nseg * kBlocksPerSegment * (sizeof(Block) + 4) +
                             total_slots() * 4;

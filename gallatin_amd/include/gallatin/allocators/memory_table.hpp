@@ -93,14 +93,20 @@ enum control_word : uint32_t {
   kStatSites = 9,
   kCtlWords = 32
 };
+// Each control word has a 128-byte line of its own: the class locks, the
+// free-segment gauge and the activity counters are all updated by the rare
+// paths of many waves at once, and atomics to one line are performed one
+// after another.
+constexpr uint32_t kCtlStride = 32;  // u32 words per control word's line
+__host__ __device__ inline uint32_t* ctl_word(uint32_t* control, uint32_t w) { return control + w * kCtlStride; }
 
 #if defined(__HIPCC__)
 __device__ inline void count_error_at(uint32_t* control, uint32_t site) {
-  atom::add(&control[kCtlErrors], 1u);
-  atom::add(&control[kCtlSiteBase + site], 1u);
+  atom::add(ctl_word(control, kCtlErrors), 1u);
+  atom::add(ctl_word(control, kCtlSiteBase + site), 1u);
 }
 __device__ inline void count_stat(uint32_t* control, uint32_t stat, uint32_t n = 1u) {
-  atom::add(&control[kCtlStatBase + stat], n);
+  atom::add(ctl_word(control, kCtlStatBase + stat), n);
 }
 #endif
 
