@@ -367,6 +367,48 @@ struct Gallatin {
     index = (uint32_t)my;
   }
 
+  // The same walk with the lanes of `group` loading consecutive positions of
+  // it at once (lane of rank r: position base + r). The first position where
+  // the sequential walk stops is found with ballots (an entry holding a
+  // block; the walk's last position; a busy entry seen and a full cycle
+  // walked), so every lane of the group gets the sequential walk's result. A
+  // group of one lane walks one position at a time. Called by all lanes of
+  // `looking`; each group walks its own class.
+  __device__ void pinned_slot_group(uint32_t tree, uint32_t hint, uint64_t group, uint64_t looking,
+                                    uint32_t& index, uint32_t& entry) const {
+    const uint32_t n = pinned_slots_of(tree);
+    const uint32_t original = hint % n;
+    const uint32_t cap = (uint32_t)kPinnedWalk;
+    const uint32_t last = original >= 1 && n - 1 < cap ? n - 1 : cap;  // the walk's last position
+    const uint32_t g = wave::popc(group);
+    const uint32_t rank = wave::popc(group & wave::lanemask_lt());
+    const uint64_t at_or_below = wave::lanemask_lt() | (1ull << wave::lane_id());
+    bool saw_busy = false;
+    bool found = ((looking >> wave::lane_id()) & 1ull) == 0;  // lanes outside `looking` only vote
+    index = original;
+    entry = 0u;
+    for (uint32_t base = 0; wave::ballot(!found) & looking; base += g) {
+      const uint32_t k = base + rank;
+      const bool in = !found && k <= last;
+      const uint32_t pos = (original + k) % n;
+      const uint32_t e = in ? atom::load(slot_ptr(tree, pos)) : 0u;
+      const uint64_t busy_m = wave::ballot(in && e == kSlotBusy) & group;
+      const bool busy_so_far = saw_busy || (busy_m & at_or_below) != 0;
+      const bool stop = in && ((e != 0u && e != kSlotBusy) || k == last || (busy_so_far && k + 1 >= n));
+      const uint64_t stop_m = wave::ballot(stop) & group;
+      const int src = stop_m ? wave::first_lane(stop_m) : (int)wave::lane_id();
+      const uint32_t s_pos = wave::shfl(pos, src);
+      const uint32_t s_e = wave::shfl(e, src);
+      const uint32_t s_busy = wave::shfl((uint32_t)busy_so_far, src);
+      if (!found && stop_m) {
+        index = s_pos;
+        entry = (s_busy && (s_e == 0u || s_e == kSlotBusy)) ? kSlotBusy : s_e;
+        found = true;
+      }
+      saw_busy = saw_busy || busy_m != 0;
+    }
+  }
+
   // ------------------------------------------------------- slice claims
   // One claim pass for the lanes of `looking` (the caller's exec mask),
   // reference gallatin.cuh:865-1078. Lanes asking for the same class form a
@@ -396,6 +438,7 @@ struct Gallatin {
     // slot there is filled unless the class is out of memory. Otherwise (and
     // always for a single lane) the reference's walk over the pinned slots.
     bool placed = false;
+    bool dry = false;  // the leader found the class out of memory in this pass
     if (attempt == 0 && usable > pinned_slots_of(tree) && (looking & (looking - 1)) != 0) {
       const uint32_t si = lead_hint % usable;
       const uint32_t se = atom::load(slot_ptr(tree, si));
@@ -403,15 +446,29 @@ struct Gallatin {
         index = si;
         entry = se;
         placed = true;
-      } else if (se == 0u && !class_out_of_memory(tree, tid)) {
-        index = si;
-        entry = 0u;
-        placed = true;
+      } else if (se == 0u) {
+        dry = class_out_of_memory(tree, tid);
+        if (!dry) {
+          index = si;
+          entry = 0u;
+          placed = true;
+        }
       }
     }
-    if (!placed) pinned_slot(tree, lead_hint, index, entry);
-    index = wave::shfl(index, leader);
-    entry = wave::shfl(entry, leader);
+    placed = wave::shfl((uint32_t)placed, leader) != 0u;
+    dry = wave::shfl((uint32_t)dry, leader) != 0u;
+    if (placed) {
+      index = wave::shfl(index, leader);
+      entry = wave::shfl(entry, leader);
+    }
+    if (wave::ballot(!placed) & looking) {
+      uint32_t wi = 0u, we = 0u;
+      pinned_slot_group(tree, lead_hint, group, looking & ~wave::ballot(placed), wi, we);
+      if (!placed) {
+        index = wi;
+        entry = we;
+      }
+    }
     // Every pinned slot of the class busy: the group's lanes each look at
     // one overflow slot, from a start that moves with the hint and the pass,
     // and take the first holding a block, else the first empty one (which
@@ -502,7 +559,8 @@ struct Gallatin {
       if ((int)me == wave::first_lane(todo)) {
         const uint64_t lead_tid = tid;
         if (empty) {
-          refilled = !class_out_of_memory(tree, lead_tid) && refill_slot(tree, index, lead_tid);
+          // `dry`: this pass's spread check already ran the same test
+          refilled = !dry && !class_out_of_memory(tree, lead_tid) && refill_slot(tree, index, lead_tid);
         } else {
           if (replace) replace_block(tree, index, block, lead_tid);
           if (completed) recycle_block(block);  // the write-off finished the block
