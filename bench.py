@@ -280,25 +280,29 @@ def main():
     log(f"[bench] creating Gallatin<16MiB,16,4096> pool={pool} threads/round={threads} rounds={rounds}")
     g = ga.Gallatin.generate_on_device(pool, args.seed)
 
-    def step(seed, tag):
-        """One step: reset, every round (malloc kernel + free kernel), drain."""
+    def step(seed, tag, rare_paths=False):
+        """One step: reset, every round (malloc kernel + free kernel), drain.
+        rare_paths: also read the allocator's rare-path counters after each
+        round for the log (one more device read per round; warm-up only)."""
         g.mixed_reset(threads, rounds, seed)
         total = None
-        before = g.stats()
+        before = g.stats() if rare_paths else None
         for r in range(rounds):
             res = g.mixed_round(r)
-            now = g.stats()
-            delta = {k: now[k] - before[k] for k in now if now[k] != before[k]}
-            before = now
-            log(f"[bench] {tag} round {r}: mallocs {res['mallocs']} frees {res['frees']} misses {res['misses']} "
-                f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f} rare paths {delta}")
+            line = (f"[bench] {tag} round {r}: mallocs {res['mallocs']} frees {res['frees']} misses {res['misses']} "
+                    f"malloc_ms {res['malloc_kernel_ms']:.2f} free_ms {res['free_kernel_ms']:.2f}")
+            if rare_paths:
+                now = g.stats()
+                line += f" rare paths { {k: now[k] - before[k] for k in now if now[k] != before[k]} }"
+                before = now
+            log(line)
             total = res if total is None else {k: total[k] + res[k] for k in total}
         res = g.mixed_drain()
         total = {k: total[k] + res[k] for k in total}
         return total
 
     for w in range(args.warmup):
-        r = step(args.seed + 1000 + w, f"warmup {w}")
+        r = step(args.seed + 1000 + w, f"warmup {w}", rare_paths=True)
         log(f"[bench] warmup {w}: {r}")
 
     results = []

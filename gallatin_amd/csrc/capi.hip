@@ -36,6 +36,7 @@ struct alignas(128) counter_stripe {
 };
 struct run_counters {
   counter_stripe s[kCounterStripes];
+  uint32_t errors_before, errors_after;  // the allocator's error count around the run (copied on the stream)
 };
 
 constexpr int kEventPairs = 96;
@@ -854,20 +855,25 @@ template <class F>
 int timed_run(gal_allocator* h, gal_run_result* out, F&& body) {
   gal_run_result r;
   std::memset(&r, 0, sizeof(r));
-  uint64_t err0 = 0, err1 = 0;
-  if (read_errors(h, &err0, nullptr) != GAL_OK) return GAL_ERR_HIP;
-  if (hipMemsetAsync(h->d_counters, 0, sizeof(run_counters), h->stream) != hipSuccess) return GAL_ERR_HIP;
+  // the error word is copied before and after the kernels on the same stream,
+  // so that one read after the run returns both with the counters
+  uint32_t* errors = ga::ctl_word(h->control, ga::kCtlErrors);
+  if (hipMemsetAsync(h->d_counters, 0, sizeof(run_counters), h->stream) != hipSuccess ||
+      hipMemcpyAsync(&h->d_counters->errors_before, errors, 4, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return GAL_ERR_HIP;
   launch_timer t(h);
   int rc = body(t);
   if (rc != GAL_OK) {
     (void)hipStreamSynchronize(h->stream);
     return rc;
   }
+  if (hipMemcpyAsync(&h->d_counters->errors_after, errors, 4, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return GAL_ERR_HIP;
   rc = t.finish(&r);
   if (rc != GAL_OK) return rc;
   run_counters c;
   if (hipMemcpy(&c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
-  if (read_errors(h, &err1, nullptr) != GAL_OK) return GAL_ERR_HIP;
+  const uint64_t err0 = c.errors_before, err1 = c.errors_after;
   for (const counter_stripe& k : c.s) {
     r.mallocs += k.mallocs;
     r.frees += k.frees;
