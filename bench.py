@@ -80,8 +80,8 @@ def forms(seed):
         with ga.Batch(g, n) as b:
             b.generate(n, seed=seed, size_classes=9)
             for name, form, width, run in (("per_thread", ga.FORM_PER_THREAD, 64, 1),
-                                           ("batched", ga.FORM_BATCHED, 64, 16),
-                                           ("graph", ga.FORM_GRAPH, 64, 16)):
+                                           ("batched", ga.FORM_BATCHED, 64, 32),
+                                           ("graph", ga.FORM_GRAPH, 64, 32)):
                 b.run(form, width=width, run=run, replays=1)  # warm-up
                 r = b.run(form, width=width, run=run, replays=5, verify=ga.GAL_VERIFY_STORE)
                 out[name] = {"mallocs_frees_per_s": rate(r["mallocs"] + r["frees"], r["total_ms"]),
@@ -192,7 +192,7 @@ def slab(seed):
 
     pool = 32 << 30
     best = autotune.tune(pool, slab_keys=1 << 22, slab_buckets=1 << 16, batch_requests=1 << 22, size_classes=13,
-                         replays=2, widths=(16, 64), runs=(1, 16))
+                         replays=2, widths=(16, 64), runs=(16, 32))
     seg, mn, mx = best["triple"]
     g = ga.Gallatin.generate_on_device(pool, seed, segment_size=seg, min_size=mn, max_size=mx)
     try:

@@ -127,7 +127,7 @@ __global__ void bin_scatter_kernel(const uint64_t* sizes, uint64_t n, const uint
 // them with one free_run; otherwise it mallocs and frees them one by one.
 // Lanes of one class are split into claim groups of 2^width_log2 lanes, each
 // with its own slot hint.
-constexpr uint32_t kMaxRun = 16;
+constexpr uint32_t kMaxRun = 64;
 // A lane holds at most this many bytes of one run at once: the persistent
 // grid keeps thousands of waves on neighbouring spans of one bin, so runs of
 // large requests would otherwise hold far more memory than the pool has.
@@ -136,6 +136,46 @@ constexpr uint64_t kRunBytes = 64 * 1024;
 // sleeps, before its requests count as misses: in a batch most of the
 // memory is held by requests of the same launch that are about to be freed.
 constexpr int kBatchRetries = 12;
+
+// Tags of up to kTagChunk requests of one bulk claim, checked with their
+// memory operations issued together (the per-thread form issues them from
+// different threads): in store mode all tags are stored, then all read back;
+// on free all are read, then all cleared. Atomic mode exchanges them.
+constexpr uint32_t kTagChunk = 4;
+__device__ inline uint32_t tags_in(uint64_t* const* p, const uint64_t* tag, uint32_t m, uint32_t verify) {
+  uint32_t bad = 0;
+  if (verify == GAL_VERIFY_ATOMIC) {
+    uint64_t old[kTagChunk];
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++) old[j] = j < m ? atom::exch(p[j], tag[j]) : 0ull;
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++) bad += j < m && old[j] != 0ull;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++)
+      if (j < m) *(volatile uint64_t*)p[j] = tag[j];
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++) bad += j < m && *(volatile uint64_t*)p[j] != tag[j];
+  }
+  return bad;
+}
+__device__ inline uint32_t tags_out(uint64_t* const* p, const uint64_t* tag, uint32_t m, uint32_t verify) {
+  uint32_t bad = 0;
+  uint64_t got[kTagChunk];
+  if (verify == GAL_VERIFY_ATOMIC) {
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++) got[j] = j < m ? atom::exch(p[j], 0ull) : 0ull;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++) got[j] = j < m ? *(volatile uint64_t*)p[j] : 0ull;
+#pragma unroll
+    for (uint32_t j = 0; j < kTagChunk; j++)
+      if (j < m) *(volatile uint64_t*)p[j] = 0ull;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kTagChunk; j++) bad += j < m && got[j] != tag[j];
+  return bad;
+}
 
 __device__ inline void batch_backoff(int t) {
   if (t < 4)
@@ -159,11 +199,15 @@ __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes,
   for (uint64_t sp = global_wave(); sp < spans; sp += waves) {
     const uint64_t first = sp * span + (uint64_t)lane * run;
     const uint32_t k = first < n ? (uint32_t)(n - first < run ? n - first : run) : 0u;
-    // one bulk claim when the lane's requests share a slice class
+    // one bulk claim when the lane's requests share a slice class: the
+    // binned order is sorted by bin, so they do exactly when the first and
+    // the last share a slice bin
     const uint64_t size0 = k ? sizes[perm[first]] : 0;
+    const uint64_t size_last = k > 1 ? sizes[perm[first + k - 1]] : size0;
     const uint64_t stride = k ? A::run_stride(size0) : 0;
-    bool bulk = (mode & kBatchMalloc) && k > 1 && stride != 0;
-    for (uint32_t r = 1; r < k && bulk; r++) bulk = A::run_stride(sizes[perm[first + r]]) == stride;
+    const uint32_t bin0 = request_bin<A>(size0);
+    const bool bulk = (mode & kBatchMalloc) && k > 1 && stride != 0 && bin0 < kBins - 1 &&
+                      request_bin<A>(size_last) == bin0;
     if (bulk) {
       // sub-runs of at most kRunBytes
       const uint32_t cap = stride >= kRunBytes ? 1u : (uint32_t)(kRunBytes / stride);
@@ -178,12 +222,24 @@ __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes,
                         : (char*)a->malloc_at(size0, tid, hint, sub);
         }
         if (base) {
-          for (uint32_t r = 0; r < kk; r++) {
-            uint64_t* p = (uint64_t*)(base + r * stride);
-            const uint32_t req = perm[first + r0 + r];
-            nb += !tag_in(p, gw::payload_tag(seed, req), verify);
-            if (mode & kBatchFree) nb += !tag_out(p, gw::payload_tag(seed, req), verify);
-            if (ptrs) ptrs[req] = (mode & kBatchFree) ? nullptr : p;
+#pragma unroll 1
+          for (uint32_t c0 = 0; c0 < kk; c0 += kTagChunk) {
+            const uint32_t m = kk - c0 < kTagChunk ? kk - c0 : kTagChunk;
+            uint32_t req[kTagChunk];
+            uint64_t* p[kTagChunk];
+            uint64_t tag[kTagChunk];
+#pragma unroll
+            for (uint32_t j = 0; j < kTagChunk; j++) {
+              req[j] = j < m ? perm[first + r0 + c0 + j] : 0u;
+              p[j] = (uint64_t*)(base + (uint64_t)(c0 + j) * stride);
+              tag[j] = gw::payload_tag(seed, req[j]);
+            }
+            nb += tags_in(p, tag, m, verify);
+            if (mode & kBatchFree) nb += tags_out(p, tag, m, verify);
+            if (ptrs)
+#pragma unroll
+              for (uint32_t j = 0; j < kTagChunk; j++)
+                if (j < m) ptrs[req[j]] = (mode & kBatchFree) ? nullptr : p[j];
           }
           nm += kk;
           if (mode & kBatchFree) {

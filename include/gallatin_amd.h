@@ -256,7 +256,7 @@ int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n);
  * A batch holds up to `capacity` request sizes. Serving it bins the requests
  * by the class they claim from (a stable counting sort on the device), then
  * one persistent launch serves the binned order, `run` consecutive requests
- * per lane (1..16): a lane whose requests share a slice class claims them
+ * per lane (1..64): a lane whose requests share a slice class claims them
  * with one bulk claim (Gallatin::malloc_run_at: consecutive slices of one
  * block, the lane's share of one group add) and frees them with one
  * free_run. Lanes of one class form claim groups of `width` lanes (1..64, a
