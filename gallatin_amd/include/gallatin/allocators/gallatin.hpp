@@ -344,36 +344,18 @@ struct Gallatin {
     return sub_trees[tree].find_random_valid_index(tid) == utils::kFail && segment_tree.nothing_to_claim();
   }
 
-  // the pinned slot a group uses: start at hint % n, step past empty slots
-  // (at most 30 steps, reference shared_block_storage.cuh:119-145); returns
-  // the slot's index and the entry read there
-  // The reference's walk over the pinned slots. Once a busy slot has been
-  // seen the walk ends after one cycle, reporting busy (the caller then
-  // tries the overflow slots); a single lane never sees a busy slot.
-  __device__ void pinned_slot(uint32_t tree, uint32_t hint, uint32_t& index, uint32_t& entry) const {
-    const int n = (int)pinned_slots_of(tree);
-    int my = (int)(hint % (uint32_t)n);
-    const int original = my;
-    entry = atom::load(slot_ptr(tree, (uint32_t)my));
-    bool saw_busy = entry == kSlotBusy;
-    for (int counter = 0; (entry == 0u || entry == kSlotBusy) && my != original - 1;) {
-      if (saw_busy && counter + 1 >= n) break;
-      my = (my + 1) % n;
-      entry = atom::load(slot_ptr(tree, (uint32_t)my));
-      saw_busy |= entry == kSlotBusy;
-      if (++counter >= kPinnedWalk) break;
-    }
-    if (saw_busy && (entry == 0u || entry == kSlotBusy)) entry = kSlotBusy;
-    index = (uint32_t)my;
-  }
-
-  // The same walk with the lanes of `group` loading consecutive positions of
-  // it at once (lane of rank r: position base + r). The first position where
-  // the sequential walk stops is found with ballots (an entry holding a
-  // block; the walk's last position; a busy entry seen and a full cycle
-  // walked), so every lane of the group gets the sequential walk's result. A
-  // group of one lane walks one position at a time. Called by all lanes of
-  // `looking`; each group walks its own class.
+  // The pinned slot a group uses, by the reference's walk
+  // (shared_block_storage.cuh:119-145): start at hint % n and step past slots
+  // that are empty or busy, at most 30 steps, ending after a full cycle once a
+  // busy slot has been seen (then reporting busy: the caller tries the
+  // overflow slots; a single lane never sees a busy slot); returns the slot's
+  // index and the entry read there. The lanes of `group` load consecutive
+  // positions of the walk at once (lane of rank r: position base + r) and
+  // the first position where the sequential walk stops is found with ballots
+  // (an entry holding a block; the walk's last position; a busy entry seen
+  // and a full cycle walked), so every lane of the group gets the sequential
+  // walk's result. A group of one lane walks one position at a time. Called
+  // by all lanes of `looking`; each group walks its own class.
   __device__ void pinned_slot_group(uint32_t tree, uint32_t hint, uint64_t group, uint64_t looking,
                                     uint32_t& index, uint32_t& entry) const {
     const uint32_t n = pinned_slots_of(tree);

@@ -1,6 +1,6 @@
 """The pinned-slot walk done by a group's lanes together
 (Gallatin::pinned_slot_group) must stop where the reference's sequential walk
-(shared_block_storage.cuh:119-145, restated as Gallatin::pinned_slot and in
+(shared_block_storage.cuh:119-145, restated in
 the oracle) stops: same slot, same entry, for every slot count, start and
 group size. Both walks are restated here on random slot states, including
 all-empty, all-busy and wrapping starts; a single-lane group is the
