@@ -151,6 +151,38 @@ def test_plain_kernel_sizes(size):
         g.free_on_device()
 
 
+SPREAD_CASES = [
+    # (segment, min, max), pool, size: the 1 GiB pool gives the 128 B, 512 B
+    # and 2 KiB classes 64, 32 and 8 usable slots (more than their pinned
+    # ones); on the 64 MiB pool the budget leaves every class at its pinned
+    # slots
+    ((16 * MiB, 16, 4096), 1 << 30, 128),
+    ((16 * MiB, 16, 4096), 1 << 30, 512),
+    ((16 * MiB, 16, 4096), 1 << 30, 2048),
+    ((MiB, 16, 256), 64 * MiB, 64),
+]
+
+
+@pytest.mark.parametrize("cfg,pool,size", SPREAD_CASES, ids=[f"{c[1] >> 20}MiB-{c[2]}B" for c in SPREAD_CASES])
+def test_spread_claims_leave_nothing_stranded(cfg, pool, size):
+    """Under load a class claims from up to its usable slots (pinned and
+    overflow, within 1/16 of the pool in blocks): repeated kernels of 2^18
+    concurrent malloc+free pairs never miss, clash or leak, and leave the
+    metadata quiescent after each run."""
+    seg, mn, mx = cfg
+    say(f"[spread] pool={pool} cfg={cfg} size={size}")
+    g = ga.Gallatin.generate_on_device(pool, 5, segment_size=seg, min_size=mn, max_size=mx)
+    try:
+        for rep in range(3):
+            r = g.run_plain(1 << 18, size, 40 + rep, verify=ga.GAL_VERIFY_ATOMIC)
+            assert r["mallocs"] == 1 << 18 and r["misses"] == 0, r
+            assert r["tag_errors"] == 0 and r["alloc_errors"] == 0, r
+            invariants.check_quiescent(g.snapshot(), seg, mn)
+        assert g.errors() == (0, 0)
+    finally:
+        g.free_on_device()
+
+
 @pytest.mark.parametrize("verify", [ga.GAL_VERIFY_ATOMIC, ga.GAL_VERIFY_STORE])
 def test_mixed_rounds_invariants(verify):
     """Atomic mode: tags exchanged in and out (sharing detected), frees with
