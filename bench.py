@@ -99,7 +99,7 @@ def forms(seed):
     g = ga.Gallatin.generate_on_device(200 << 30, seed)
     try:
         for name, med, cap, n, slots, window in (("serving", 1024, 32768, 1 << 19, 65536, 0.1),
-                                                  ("serving_short", 16, 1024, 1 << 22, 65536, 0.05)):
+                                                  ("serving_short", 16, 1024, 1 << 23, 65536, 0.05)):
             reqs = ga.generate_requests(n, 0.0, median_pages=med, sigma=1.0, max_pages=cap, page_bytes=0,
                                         size_classes=9, seed=seed)
             with ga.Serving(g, len(reqs)) as s:

@@ -96,15 +96,16 @@ struct gal_serve {
   gal_allocator* h = nullptr;
   uint64_t capacity = 0;
   uint64_t n = 0;
+  // requests bin after bin; the per-request outputs follow the same order
   void* d_reqs = nullptr;  // serve_req[capacity]
   uint64_t* d_checksums = nullptr;
   uint64_t* d_admit = nullptr;  // wall-clock ticks after the start, ~0: aborted
   uint64_t* d_done = nullptr;
   void* d_state = nullptr;                // serve_state
   unsigned long long* d_hist = nullptr;   // per-page malloc latency histogram
-  std::vector<uint64_t> arrival_ns;       // host copy of the arrivals
+  std::vector<uint64_t> arrival_ns;       // host copy of the arrivals, in device order
   std::vector<uint32_t> page_sizes;       // distinct page sizes of the loaded requests
-  uint32_t* d_order = nullptr;            // request indices by bin
+  std::vector<uint32_t> order;            // caller's index of the request at each device position
   std::vector<uint32_t> bin_start, bin_count;  // per non-empty bin
   std::vector<double> bin_work;           // pages grown and drained per bin (for dealing waves)
   uint64_t max_need = 0;                  // largest request, bytes
@@ -1510,7 +1511,7 @@ int gal_serve_generate(const gal_serve_workload* w, gal_request* out) {
 int gal_serve_destroy(gal_serve* s) {
   if (!s) return GAL_ERR_INVALID;
   if (s->h) (void)hipStreamSynchronize(s->h->stream);
-  void* bufs[] = {s->d_reqs, s->d_checksums, s->d_admit, s->d_done, s->d_state, s->d_hist, s->d_order};
+  void* bufs[] = {s->d_reqs, s->d_checksums, s->d_admit, s->d_done, s->d_state, s->d_hist};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1528,7 +1529,6 @@ int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out) {
                   hipMalloc(&s->d_admit, capacity * 8) == hipSuccess &&
                   hipMalloc(&s->d_done, capacity * 8) == hipSuccess &&
                   hipMalloc(&s->d_state, sizeof(serve_state)) == hipSuccess &&
-                  hipMalloc(&s->d_order, capacity * 4) == hipSuccess &&
                   hipMalloc(&s->d_hist, kHistBuckets * sizeof(unsigned long long)) == hipSuccess;
   if (!ok) {
     gal_serve_destroy(s);
@@ -1540,25 +1540,15 @@ int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out) {
 
 int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n) {
   if (!s || !reqs || n == 0 || n > s->capacity) return GAL_ERR_INVALID;
-  std::vector<serve_req> dev(n);
-  s->arrival_ns.resize(n);
   s->page_sizes.clear();
   s->max_need = 0;
-  const double tick_ns = s->h->tick_ns;
   for (uint64_t i = 0; i < n; i++) {
     if (reqs[i].pages == 0 || reqs[i].page_bytes < 16) return GAL_ERR_INVALID;
-    dev[i].arrival = (uint64_t)((double)reqs[i].arrival_ns / tick_ns);
-    dev[i].id = reqs[i].id;
-    dev[i].pages = reqs[i].pages;
-    dev[i].page_bytes = reqs[i].page_bytes;
-    s->arrival_ns[i] = reqs[i].arrival_ns;
     if (std::find(s->page_sizes.begin(), s->page_sizes.end(), reqs[i].page_bytes) == s->page_sizes.end())
       s->page_sizes.push_back(reqs[i].page_bytes);
     const uint64_t need = (uint64_t)reqs[i].pages * reqs[i].page_bytes;
     if (need > s->max_need) s->max_need = need;
   }
-  if (hipMemcpy(s->d_reqs, dev.data(), n * sizeof(serve_req), hipMemcpyHostToDevice) != hipSuccess)
-    return GAL_ERR_HIP;
   // stable counting sort by bin; empty bins dropped
   std::vector<uint32_t> raw(n), counts(kServeBins, 0);
   std::vector<double> work(kServeBins, 0.0);
@@ -1567,23 +1557,35 @@ int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n) {
     counts[raw[i]]++;
     work[raw[i]] += 2.0 * reqs[i].pages + 2.0;
   }
-  std::vector<uint32_t> compact(kServeBins, 0), next(kServeBins, 0);
+  std::vector<uint32_t> next(kServeBins, 0);
   s->bin_start.clear();
   s->bin_count.clear();
   s->bin_work.clear();
   uint32_t pos = 0;
   for (uint32_t b = 0; b < kServeBins; b++) {
     if (!counts[b]) continue;
-    compact[b] = (uint32_t)s->bin_start.size();
     next[b] = pos;
     s->bin_start.push_back(pos);
     s->bin_count.push_back(counts[b]);
     s->bin_work.push_back(work[b]);
     pos += counts[b];
   }
-  std::vector<uint32_t> order(n);
-  for (uint64_t i = 0; i < n; i++) order[next[raw[i]]++] = (uint32_t)i;
-  if (hipMemcpy(s->d_order, order.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) return GAL_ERR_HIP;
+  s->order.assign(n, 0);
+  for (uint64_t i = 0; i < n; i++) s->order[next[raw[i]]++] = (uint32_t)i;
+  // the device copy in that order
+  std::vector<serve_req> dev(n);
+  s->arrival_ns.resize(n);
+  const double tick_ns = s->h->tick_ns;
+  for (uint64_t p = 0; p < n; p++) {
+    const gal_request& q = reqs[s->order[p]];
+    dev[p].arrival = (uint64_t)((double)q.arrival_ns / tick_ns);
+    dev[p].id = q.id;
+    dev[p].pages = q.pages;
+    dev[p].page_bytes = q.page_bytes;
+    s->arrival_ns[p] = q.arrival_ns;
+  }
+  if (hipMemcpy(s->d_reqs, dev.data(), n * sizeof(serve_req), hipMemcpyHostToDevice) != hipSuccess)
+    return GAL_ERR_HIP;
   s->n = n;
   s->ran = false;
   return GAL_OK;
@@ -1635,7 +1637,6 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
   // each while there are enough
   serve_bins bins;
   std::memset(&bins, 0, sizeof(bins));
-  bins.order = s->d_order;
   bins.nbins = (uint32_t)s->bin_start.size();
   const uint32_t waves = blocks * 4;
   {
@@ -1763,16 +1764,21 @@ int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom,
 int gal_serve_results(gal_serve* s, uint64_t* checksums, uint64_t* admit_ns, uint64_t* done_ns, uint64_t cap) {
   if (!s || !s->ran || cap < s->n) return GAL_ERR_INVALID;
   if (hipStreamSynchronize(s->h->stream) != hipSuccess) return GAL_ERR_HIP;
-  if (checksums && hipMemcpy(checksums, s->d_checksums, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess)
-    return GAL_ERR_HIP;
   const double tick_ns = s->h->tick_ns;
-  auto times = [&](uint64_t* dst, const uint64_t* src) {
+  std::vector<uint64_t> tmp(s->n);
+  // device order -> the caller's order; times from ticks to ns
+  auto fetch = [&](uint64_t* dst, const uint64_t* src, bool ticks) {
     if (!dst) return true;
-    if (hipMemcpy(dst, src, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess) return false;
-    for (uint64_t i = 0; i < s->n; i++) dst[i] = dst[i] == ~0ull ? ~0ull : (uint64_t)((double)dst[i] * tick_ns);
+    if (hipMemcpy(tmp.data(), src, s->n * 8, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    for (uint64_t p = 0; p < s->n; p++) {
+      const uint64_t v = tmp[p];
+      dst[s->order[p]] = !ticks || v == ~0ull ? v : (uint64_t)((double)v * tick_ns);
+    }
     return true;
   };
-  if (!times(admit_ns, s->d_admit) || !times(done_ns, s->d_done)) return GAL_ERR_HIP;
+  if (!fetch(checksums, s->d_checksums, false) || !fetch(admit_ns, s->d_admit, true) ||
+      !fetch(done_ns, s->d_done, true))
+    return GAL_ERR_HIP;
   return GAL_OK;
 }
 

@@ -41,6 +41,13 @@ constexpr uint32_t kFlushEvery = 16;
 constexpr uint32_t kAdmitBackoffMin = 8;
 constexpr uint32_t kAdmitBackoffMax = 64;
 constexpr uint32_t kFreeBatch = 16;  // lanes holding a drained claim before the wave frees them
+// Idle lanes take their tickets together: once kTicketBatch of them are
+// idle, once the oldest has waited kTicketWait iterations, or when no lane
+// of the wave is busy. Taking a ticket, admitting and claiming put dependent
+// atomics and loads on the wave's critical path; with short sequences some
+// lane finished in almost every iteration and every iteration paid for them.
+constexpr uint32_t kTicketBatch = 16;
+constexpr uint32_t kTicketWait = 16;
 
 struct serve_req {
   uint64_t arrival;  // wall-clock ticks after the start
@@ -67,15 +74,16 @@ struct serve_state {
 };
 
 // Requests are binned by the class their pages claim from: each bin is a
-// queue in arrival order with a ticket counter of its own, and each wave
+// queue in arrival order with a ticket counter of its own (the requests are
+// stored bin after bin, so a ticket is the request's position and taking one
+// is a single dependent load after the counter's add), and each wave
 // serves one bin (waves dealt to bins in proportion to their work), so the
 // growing lanes of a wave claim as one group and its draining lanes return
 // pages of the same blocks. A wave whose bin has handed out every ticket
 // moves to the next bin; once it has found every bin empty it leaves.
 constexpr uint32_t kServeBins = 32;
 struct serve_bins {
-  const uint32_t* order;            // request indices, bin after bin, each in arrival order
-  uint32_t start[kServeBins];       // first position of each bin in `order`
+  uint32_t start[kServeBins];       // first position of each bin among the requests
   uint32_t count[kServeBins];       // requests of each bin
   uint32_t wave_start[kServeBins + 1];  // waves [wave_start[b], wave_start[b+1]) start on bin b
   uint32_t nbins;
@@ -186,6 +194,7 @@ __global__ void __launch_bounds__(256)
   uint32_t nw = 0;  // pages written and drained before the end of the measuring window
 
   uint32_t admit_wait = 0;                  // wave-uniform: iterations left before the next admission try
+  uint32_t idle_age = 0;                    // wave-uniform: iterations the wave's idle lanes have waited
   uint32_t admit_backoff = kAdmitBackoffMin;  // wave-uniform: the wait after the next refusal
   for (uint32_t it = 0;; it++) {
     const uint64_t now = wall_ticks() - t0;
@@ -210,17 +219,24 @@ __global__ void __launch_bounds__(256)
         q3 = q[3];
       }
     }
+    // idle lanes that take tickets in this iteration
+    uint64_t idle = wave::ballot(state == kSeqIdle);
+    if (idle && wave::popc(idle) < kTicketBatch && idle_age < kTicketWait &&
+        wave::ballot(state == kSeqWaiting || state == kSeqGrowing || state == kSeqDraining)) {
+      idle_age++;
+      idle = 0;
+    }
     uint64_t room = 0, seen = 0;
     const bool admit_now = admit_wait == 0;
     if (admit_wait) admit_wait--;
-    if (admit_now && wave::ballot(state == kSeqIdle || state == kSeqWaiting) && lane == 0) {
+    if (admit_now && (idle || wave::ballot(state == kSeqWaiting)) && lane == 0) {
       room = (uint64_t)atom::load(free_segs) * A::kBytesPerSegment;
       seen = atom::load(&st->committed.v);
     }
 
     // ---- tickets for idle lanes, from this wave's bin
-    const uint64_t idle = wave::ballot(state == kSeqIdle);
     if (idle) {
+      idle_age = 0;
       const int leader = wave::first_lane(idle);
       const uint32_t k = wave::popc(idle);
       uint64_t base = 0;
@@ -229,7 +245,7 @@ __global__ void __launch_bounds__(256)
       if (state == kSeqIdle) {
         const uint64_t t = base + wave::popc(idle & wave::lanemask_lt());
         if (t < bins.count[bin]) {
-          ticket = bins.order[bins.start[bin] + t];
+          ticket = bins.start[bin] + t;
           r = reqs[ticket];
           state = kSeqWaiting;
         }
