@@ -431,8 +431,9 @@ int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n);
  * GAL_ERR_INVALID if a request could never be admitted. */
 int gal_serve_run(gal_serve* s, uint32_t slots, uint32_t run, uint64_t headroom, double deadline_s, double window_s,
                   gal_serve_result* out);
-/* per request of the last run: checksum (0 if aborted), admission and
- * completion times in ns after the start (~0 if aborted) */
+/* per request of the last run, indexed as the requests passed to
+ * gal_serve_load: checksum (0 if aborted), admission and completion times in
+ * ns after the start (~0 if aborted) */
 int gal_serve_results(gal_serve* s, uint64_t* checksums, uint64_t* admit_ns, uint64_t* done_ns, uint64_t cap);
 
 /* Agent-scope 32-bit atomic add rate of this GPU, in returned adds per
