@@ -32,6 +32,13 @@ BYTES_FREE_SIDE = 18
 # step) scaled by what the counters read for a 1 GiB device copy (fetch x2.0,
 # write x1.0): profiles/mixed_slices_8GiB_hbm_traffic.md.
 MEASURED_TRAFFIC_BYTES = {"mixed_malloc_kernel": 203.6e6, "mixed_free_kernel": 358.3e6}
+# Agent-scope atomic requests per launch at the default workload, from a
+# rocprofv3 TCC_ATOMIC_sum run (averaged over a step). Every such atomic is
+# performed beyond the L2 (TCC_EA0_ATOMIC_sum reads the same). One request
+# serves the lanes of one instruction that fall in one 64-B span (16 lanes on
+# consecutive 4-B words); a single-lane atomic is one request:
+# profiles/mixed_slices_8GiB_tcc_atomics.md.
+MEASURED_ATOMIC_REQUESTS = {"mixed_malloc_kernel": 324070, "mixed_free_kernel": 205989}
 DEFAULT_WORKLOAD = (8, 22, 10)  # pool GiB, log2 threads per round, rounds
 
 
@@ -395,6 +402,24 @@ def main():
         "wave_hot_65536_words": ga.atomic_rate(ga.ATOMIC_WAVE_HOT, 1 << 22, 16, 65536),
     }
     log(f"[bench] atomic rates {out['atomic_rates_per_s']}")
+    if default:
+        # the same kernel against the atomic request rate this GPU sustains on
+        # distinct words (the probe's lane rate / 16 lanes per request) and,
+        # for one hot line, the single-line rate
+        dom = out["roofline"]["kernel"]
+        reqs = MEASURED_ATOMIC_REQUESTS[dom]
+        achieved_req = reqs / (out["roofline"]["kernel_avg_us"] * 1e-6)
+        peak_req = out["atomic_rates_per_s"]["distinct_lanes"] / 16
+        out["atomic_roofline"] = {
+            "bound": "agent-scope atomics",
+            "kernel": dom,
+            "achieved": achieved_req,
+            "peak": peak_req,
+            "unit": "requests/s",
+            "frac": achieved_req / peak_req,
+            "requests_per_launch": reqs,
+            "one_line_peak": out["atomic_rates_per_s"]["wave_hot_4_words"] / 4,
+        }
 
     if not args.headline_only:
         out["forms"] = forms(args.seed)
