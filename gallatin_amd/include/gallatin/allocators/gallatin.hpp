@@ -613,7 +613,18 @@ struct Gallatin {
   // claim starts at together (veb_tree::gather_plan), which reads the tree
   // one word per lane per round trip; one lane holds the lock and claims.
   // With one lane the result is the reference's.
-  __device__ __noinline__ uint64_t malloc_segment_allocation(uint64_t n, uint64_t lanes) {
+  //
+  // The same search first runs without the lock: when the tree as read now
+  // holds no run of n free segments the request fails at once (`oom`, a
+  // further attempt would repeat it). The lock orders claimers only (frees
+  // do not take it), and a run that another claimer holds for a moment is
+  // claimed at that moment, so this is a failure the locked search could
+  // also give; with one lane the tree is the same either way and the locked
+  // search fails without changing anything. Under heavy out-of-memory
+  // traffic failing requests no longer queue for the lock.
+  __device__ __noinline__ uint64_t malloc_segment_allocation(uint64_t n, uint64_t lanes, bool& oom) {
+    oom = segment_tree.gather_plan(n, lanes) == utils::kFail;
+    if (oom) return utils::kFail;
     const int leader = wave::first_lane(lanes);
     uint32_t locked = 0;
     if ((int)wave::lane_id() == leader) {
@@ -669,9 +680,10 @@ struct Gallatin {
       } else {
         const uint64_t runs = wave::readlane64(n, who);
         uint64_t offset = utils::kFail;
+        bool oom = false;
         if (runs + kNumTrees + 1 < 0xFFFFull)
-          for (int a = 0; a < kSegmentAttempts && offset == utils::kFail; a++)
-            offset = malloc_segment_allocation(runs, callers);
+          for (int a = 0; a < kSegmentAttempts && offset == utils::kFail && !oom; a++)
+            offset = malloc_segment_allocation(runs, callers, oom);
         if ((int)wave::lane_id() == who && offset != utils::kFail) out = offset_to_allocation(offset, 0);
       }
     }
