@@ -423,13 +423,21 @@ struct Gallatin {
     bool dry = false;  // the leader found the class out of memory in this pass
     if (attempt == 0 && usable > pinned_slots_of(tree) && (looking & (looking - 1)) != 0) {
       const uint32_t si = lead_hint % usable;
+      // read with the slot, in the same round trip: the class's tree summary
+      // and the free-segment gauge
+      const uint64_t class_top = sub_trees[tree].top_word();
+      const uint32_t free_segments = atom::load(ctl_word(control, kCtlFreeSegs));
       const uint32_t se = atom::load(slot_ptr(tree, si));
       if (se != 0u && se != kSlotBusy) {
         index = si;
         entry = se;
         placed = true;
       } else if (se == 0u) {
-        dry = class_out_of_memory(tree, tid);
+        // The class's tree empty (its summaries are exact: it changes only by
+        // insert_force_update/remove) and no free segment left is out of
+        // memory without the search: the search would find nothing either.
+        dry = (sub_trees[tree].top_looks_empty(class_top) && free_segments == 0u) ||
+              class_out_of_memory(tree, tid);
         if (!dry) {
           index = si;
           entry = 0u;

@@ -131,7 +131,11 @@ struct veb_tree {
   // true when the tree holds no item as seen from the top summary. Exact for
   // trees changed only by insert_force_update/remove (their summaries track
   // the item layer); the segment tree's runs can leave stale summaries.
-  __device__ bool looks_empty() const { return load_word(0, 0) == 0 && (num_layers > 1 || words[0] == 1); }
+  __device__ bool looks_empty() const { return top_looks_empty(top_word()); }
+  // the same test split in two, for a caller that overlaps the top word's
+  // load with other loads
+  __device__ uint64_t top_word() const { return load_word(0, 0); }
+  __device__ bool top_looks_empty(uint64_t top) const { return top == 0 && (num_layers > 1 || words[0] == 1); }
 
   // nothing malloc_first could claim: no item 0 and no successor of 0
   __device__ bool nothing_to_claim() const { return !query(0) && successor_thorough(0) == utils::kFail; }
