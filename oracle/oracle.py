@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+# ORACLE_LIB selects another build of the same sources (e.g. the sanitizer build)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "build", "liboracle.so")
 NULL = (1 << 64) - 1
 
 u64 = ctypes.c_uint64
