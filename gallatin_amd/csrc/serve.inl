@@ -3,8 +3,9 @@
 // One persistent launch serves a stream of sequence requests. Every lane of
 // the grid is a sequence slot with a small state machine:
 //
-//   idle     -> takes the next request (one ticket counter; a wave takes the
-//               tickets of all its idle lanes with one atomic)
+//   idle     -> takes the next request (one ticket counter per request bin;
+//               a wave takes the tickets of its idle lanes together, with
+//               one atomic, once enough of them are idle)
 //   waiting  -> admitted once the request has arrived (device wall clock)
 //               and the pool has room for all of its pages
 //   growing  -> one page per loop iteration: malloc, write the page header
