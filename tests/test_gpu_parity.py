@@ -67,6 +67,10 @@ CASES = [
      3000, 0.45),
     ("exhaustion", 64 * MiB, (MiB, 16, 256), [16, 32, 256, 1024, 8192, 65536, 2 * MiB], 8000, 0.35),
     ("min128", 1 << 30, (4 * MiB, 128, 1024), [16, 100, 128, 512, 1024, 3000, 200000], 6000, 0.45),
+    # 4096 segments: the segment tree and every class tree have three layers
+    # (the cases above have one or two), with segment runs that span item words
+    ("three_layers", 64 << 30, (16 * MiB, 16, 4096),
+     [16, 4096, 40000, 64 * 1024, MiB, 16 * MiB, 40 * MiB, 200 * MiB, 1100 * MiB], 3000, 0.45),
 ]
 
 
