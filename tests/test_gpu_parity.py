@@ -71,6 +71,11 @@ CASES = [
     # (the cases above have one or two), with segment runs that span item words
     ("three_layers", 64 << 30, (16 * MiB, 16, 4096),
      [16, 4096, 40000, 64 * 1024, MiB, 16 * MiB, 40 * MiB, 200 * MiB, 1100 * MiB], 3000, 0.45),
+    # the slab-hash workload's autotuned instance, and the 16 KiB-slice one
+    ("seg64_min128", 32 << 30, (64 * MiB, 128, 4096),
+     [16, 100, 128, 1000, 4096, 8192, 65536, MiB, 100 * MiB], 4000, 0.45),
+    ("seg64_max16k", 16 << 30, (64 * MiB, 16, 16384),
+     [16, 48, 1000, 16384, 20000, 70000, 5 * MiB, 70 * MiB], 4000, 0.45),
 ]
 
 
