@@ -169,6 +169,10 @@ SPREAD_CASES = [
     ((16 * MiB, 16, 4096), 1 << 30, 512),
     ((16 * MiB, 16, 4096), 1 << 30, 2048),
     ((MiB, 16, 256), 64 * MiB, 64),
+    # 64 MiB segments: the slab-hash instance's 256 B class, and a top class
+    # whose block is a whole segment (16 KiB x 4096)
+    ((64 * MiB, 128, 4096), 2 << 30, 256),
+    ((64 * MiB, 16, 16384), 16 << 30, 16384),
 ]
 
 
