@@ -1553,10 +1553,14 @@ int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out) {
 
 int gal_serve_load(gal_serve* s, const gal_request* reqs, uint64_t n) {
   if (!s || !reqs || n == 0 || n > s->capacity) return GAL_ERR_INVALID;
+  // every request is checked before anything of the previous load changes
+  for (uint64_t i = 0; i < n; i++)
+    if (reqs[i].pages == 0 || reqs[i].page_bytes < 16) return GAL_ERR_INVALID;
+  s->ran = false;
+  s->n = 0;  // nothing loaded until the copy below has succeeded
   s->page_sizes.clear();
   s->max_need = 0;
   for (uint64_t i = 0; i < n; i++) {
-    if (reqs[i].pages == 0 || reqs[i].page_bytes < 16) return GAL_ERR_INVALID;
     if (std::find(s->page_sizes.begin(), s->page_sizes.end(), reqs[i].page_bytes) == s->page_sizes.end())
       s->page_sizes.push_back(reqs[i].page_bytes);
     const uint64_t need = (uint64_t)reqs[i].pages * reqs[i].page_bytes;

@@ -175,3 +175,26 @@ def test_serving_rejects_a_request_that_never_fits():
         check_after(g)
     finally:
         g.free_on_device()
+
+
+def test_serving_invalid_load_keeps_the_previous_one():
+    """A load with an invalid request (no pages) is refused before anything
+    of the previous load changes: that load still runs, with every checksum
+    right."""
+    # 8 GiB: the default admission headroom of nine page classes (their
+    # slots' blocks and two segments each) needs about 1 GiB
+    g = ga.Gallatin.generate_on_device(8 << 30, 17)
+    try:
+        reqs = ga.generate_requests(200, 0.0, median_pages=8, sigma=1.0, max_pages=64, page_bytes=0, seed=17)
+        bad = reqs.copy()
+        bad["pages"][3] = 0
+        with ga.Serving(g, len(reqs)) as s:
+            s.load(reqs)
+            with pytest.raises(ga.GallatinError):
+                s.load(bad)
+            r = s.run(slots=256, run=8)
+            cs, _, _ = s.results()
+            check_run(r, reqs, cs)
+        check_after(g)
+    finally:
+        g.free_on_device()
