@@ -1083,6 +1083,7 @@ int gal_mixed_reset(gal_allocator* h, uint64_t threads_per_round, uint32_t round
   if (h->d_live && h->rounds_done > 0) {
     h->ops->launch_mixed_free(h, h->threads_per_round * h->rounds_done, 0, h->mixed_seed, GAL_VERIFY_STORE, true);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+    h->rounds_done = 0;  // nothing of that run is live any more, whatever fails below
   }
   // every slot of the live table is written by its round's malloc kernel
   // before any free kernel reads it, so the table needs no clearing
