@@ -245,3 +245,27 @@ def test_global_allocator():
         ga.print_global_stats()
     finally:
         ga.free_global_allocator()
+
+
+def test_mixed_reset_frees_an_unfinished_run():
+    """A reset in the middle of a mixed run frees what that run left live:
+    the metadata is quiescent right after it, and a new run starts clean."""
+    pool, threads = 2 << 30, 1 << 18
+    say(f"[mixed-reset] pool={pool} threads={threads}")
+    g = ga.Gallatin.generate_on_device(pool, 23)
+    try:
+        g.mixed_reset(threads, 6, 91)
+        for r in range(3):
+            res = g.mixed_round(r)
+            assert res["tag_errors"] == 0 and res["alloc_errors"] == 0, res
+        assert len(g.mixed_live()[0]) > 0
+        g.mixed_reset(threads, 2, 92)
+        invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+        for r in range(2):
+            res = g.mixed_round(r)
+            assert res["tag_errors"] == 0 and res["alloc_errors"] == 0, res
+        g.mixed_drain()
+        invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+        assert g.errors() == (0, 0)
+    finally:
+        g.free_on_device()
