@@ -31,7 +31,7 @@ BYTES_FREE_SIDE = 18
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch, averaged over a
 # step) scaled by what the counters read for a 1 GiB device copy (fetch x2.0,
 # write x1.0): profiles/mixed_slices_8GiB_hbm_traffic.md.
-MEASURED_TRAFFIC_BYTES = {"mixed_malloc_kernel": 198.7e6, "mixed_free_kernel": 354.7e6}
+MEASURED_TRAFFIC_BYTES = {"mixed_malloc_kernel": 196.0e6, "mixed_free_kernel": 351.6e6}
 # Agent-scope atomic requests per launch at the default workload, from a
 # rocprofv3 TCC_ATOMIC_sum run (averaged over a step). Every such atomic is
 # performed beyond the L2 (TCC_EA0_ATOMIC_sum reads the same). One request
