@@ -7,6 +7,7 @@ interface (Gallatin<SEG,MIN,MAX>::malloc/free, global_malloc/global_free) is
 the C++ header set in gallatin_amd/include/gallatin/.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -18,6 +19,7 @@ from ._capi import (  # noqa: F401
     GAL_VERIFY_ATOMIC,
     GAL_VERIFY_STORE,
     GallatinError,
+    GalGroupOp,
     GalOp,
 )
 
@@ -46,6 +48,22 @@ def ops_array(ops):
     return arr, len(ops)
 
 
+def group_ops_array(ops):
+    """ops: iterable of (kind, slot, lanes, args, tids) with args/tids indexed
+    by lane (64 entries, or a dict lane -> value) -> ctypes array of gal_group_op."""
+    ops = list(ops)
+    arr = (GalGroupOp * max(1, len(ops)))()
+    for i, (kind, slot, lanes, args, tids) in enumerate(ops):
+        arr[i].kind = kind
+        arr[i].slot = slot
+        arr[i].lanes = lanes
+        for lane in range(64):
+            if (lanes >> lane) & 1:
+                arr[i].arg[lane] = int(args[lane])
+                arr[i].tid[lane] = int(tids[lane])
+    return arr, len(ops)
+
+
 class Gallatin:
     """Host handle of one device allocator Gallatin<segment_size, min_size, max_size>."""
 
@@ -53,6 +71,11 @@ class Gallatin:
         self._h = handle
         self.info = info
         self._owned = owned
+        self._children = []  # weak references to batches, serving loops, slab tables made on it
+
+    def _adopt(self, child):
+        self._children = [c for c in self._children if c() is not None]
+        self._children.append(weakref.ref(child))
 
     # reference gallatin.cuh:324 generate_on_device(max_bytes, seed, print_info, running_calloc)
     @classmethod
@@ -97,6 +120,13 @@ class Gallatin:
 
     # reference gallatin.cuh:643 free_on_device
     def free_on_device(self):
+        # the library refuses to destroy a handle that batches, serving loops
+        # or slab tables still use (GAL_ERR_STATE): they are released first
+        for ref in self._children:
+            child = ref()
+            if child is not None:
+                child._release()
+        self._children = []
         if self._h is not None and self._owned:
             _capi.check("gal_destroy", _lib.gal_destroy(self._h))
         self._h = None
@@ -195,6 +225,14 @@ class Gallatin:
         out = (ctypes.c_uint64 * max(1, n))()
         _capi.check("gal_replay", _lib.gal_replay(self._h, arr, n, out))
         return np.frombuffer(out, dtype=np.uint64, count=n).copy()
+
+    def group_replay(self, ops):
+        """Multi-lane deterministic replay (gal_group_replay); returns an
+        (n, 64) array of per-lane results."""
+        arr, n = group_ops_array(ops)
+        out = (ctypes.c_uint64 * max(1, 64 * n))()
+        _capi.check("gal_group_replay", _lib.gal_group_replay(self._h, arr, n, out))
+        return np.frombuffer(out, dtype=np.uint64, count=64 * n).copy().reshape(n, 64)
 
     def run_plain(self, n_threads, size, seed, verify=GAL_VERIFY_STORE):
         r = _capi.GalRunResult()
@@ -372,6 +410,23 @@ def global_poison(size):
     return list(codes)
 
 
+def global_plain(n_threads, size, seed, verify=GAL_VERIFY_STORE):
+    """n_threads threads each global_malloc(size), tag, check, global_free (timed)."""
+    r = _capi.GalRunResult()
+    _capi.check("gal_global_plain", _lib.gal_global_plain(n_threads, size, seed, verify, ctypes.byref(r)))
+    return r.as_dict()
+
+
+def global_replay(ops):
+    """Single-lane replay through global_malloc/global_free; returns (results,
+    cu_id): the CU id the lane ran on is the slot the same stream replays with."""
+    arr, n = ops_array(ops)
+    out = (ctypes.c_uint64 * max(1, n))()
+    cu = ctypes.c_uint32()
+    _capi.check("gal_global_replay", _lib.gal_global_replay(arr, n, out, ctypes.byref(cu)))
+    return np.frombuffer(out, dtype=np.uint64, count=n).copy(), cu.value
+
+
 def global_smoke(n_threads, size):
     m, t = ctypes.c_uint64(), ctypes.c_uint64()
     _capi.check("gal_global_smoke", _lib.gal_global_smoke(n_threads, size, ctypes.byref(m), ctypes.byref(t)))
@@ -395,11 +450,14 @@ class Batch:
         self.allocator = allocator
         self.capacity = capacity
         self.n = 0
+        allocator._adopt(self)
 
     def close(self):
         if self._b:
             _capi.check("gal_batch_destroy", _lib.gal_batch_destroy(self._b))
             self._b = None
+
+    _release = close
 
     def __enter__(self):
         return self
@@ -485,11 +543,14 @@ class Serving:
         self._s = h
         self.allocator = allocator
         self.n = 0
+        allocator._adopt(self)
 
     def close(self):
         if self._s:
             _capi.check("gal_serve_destroy", _lib.gal_serve_destroy(self._s))
             self._s = None
+
+    _release = close
 
     def __enter__(self):
         return self
@@ -533,6 +594,7 @@ class SlabTable:
         self._t = h
         self.allocator = allocator
         self.buckets = buckets
+        allocator._adopt(self)
 
     def insert(self, n, key_seed):
         r = _capi.GalSlabResult()
@@ -550,6 +612,9 @@ class SlabTable:
             _capi.check("gal_slab_destroy", _lib.gal_slab_destroy(self._t, ctypes.byref(freed)))
             self._t = None
         return freed.value
+
+    def _release(self):
+        self.destroy()
 
     def __enter__(self):
         return self

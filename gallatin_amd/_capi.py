@@ -45,6 +45,10 @@ class GalOp(ctypes.Structure):
     _fields_ = [("kind", u32), ("slot", u32), ("arg", u64), ("tid", u64)]
 
 
+class GalGroupOp(ctypes.Structure):
+    _fields_ = [("kind", u32), ("slot", u32), ("lanes", u64), ("arg", u64 * 64), ("tid", u64 * 64)]
+
+
 class GalInfo(ctypes.Structure):
     _fields_ = [
         ("segment_size", u64),
@@ -168,6 +172,7 @@ _SIGNATURES = {
     "gal_fill": (ctypes.c_int, [vp, ctypes.POINTER(GalFillReport)]),
     "gal_snapshot": (ctypes.c_int, [vp, vp, u64, ctypes.POINTER(u64)]),
     "gal_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64)]),
+    "gal_group_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalGroupOp), u64, ctypes.POINTER(u64)]),
     "gal_errors": (ctypes.c_int, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
     "gal_error_sites": (ctypes.c_int, [vp, ctypes.POINTER(u32), u32]),
     "gal_stats": (ctypes.c_int, [vp, ctypes.POINTER(u32), u32]),
@@ -175,6 +180,8 @@ _SIGNATURES = {
     "gal_free_global_allocator": (ctypes.c_int, []),
     "gal_print_global_stats": (ctypes.c_int, []),
     "gal_global_smoke": (ctypes.c_int, [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "gal_global_plain": (ctypes.c_int, [u64, u64, u64, u32, ctypes.POINTER(GalRunResult)]),
+    "gal_global_replay": (ctypes.c_int, [ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
     "gal_create_host": (ctypes.c_int, [u64, u64, u64, u64, u64, ctypes.POINTER(vp)]),
     "gal_allocated_size": (ctypes.c_int, [u64, u64, u64, u64, ctypes.POINTER(u64)]),
     "gal_init_global_allocator_host": (ctypes.c_int, [u64, u64, ctypes.c_int]),

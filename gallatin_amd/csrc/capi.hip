@@ -66,6 +66,9 @@ struct gal_allocator {
   uint64_t lat_cap = 0;
   uint64_t lat_count = 0;
   double tick_ns = 10.0;
+  // batches, serving loops and slab tables made on this handle and not yet
+  // destroyed: they hold its stream and allocator, so the handle outlives them
+  int children = 0;
 };
 
 // a batch of requests and the device buffers that bin and serve it
@@ -123,6 +126,9 @@ struct impl_ops {
   int (*describe)(gal_allocator* h);
   int (*snapshot)(gal_allocator* h, uint8_t* out);
   int (*replay)(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t* d_results, uint32_t* d_status);
+  int (*group_replay)(gal_allocator* h, const gal_group_op* d_ops, uint64_t n, uint64_t* d_results,
+                      uint32_t* d_status);
+  uint32_t (*path_of)(uint64_t size);  // 0 slice (incl. big slices), 1 whole block, 2 segments
   void (*launch_plain)(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify);
   void (*launch_mixed_malloc)(gal_allocator* h, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
                               uint32_t verify, uint32_t* lat);
@@ -197,6 +203,69 @@ __global__ void replay_kernel(A* a, const gal_op* ops, uint64_t n, uint64_t* res
         return;
       }
       a->free(base + r);
+      results[i] = 0;
+    }
+  }
+}
+
+// One wavefront runs the group ops in order: the lanes of op i's mask reach
+// the malloc (or free) call together, with exactly that exec mask. Claims
+// use the reference's pinned-slot walk (no overflow-slot spreading), so the
+// results depend on the op stream only. results[i * 64 + l] was set to
+// GAL_NULL_OFFSET by the host for every entry; a free reads the result of an
+// earlier op of this wave, written by any of its lanes, after the
+// workgroup-scope fence that ends each op.
+template <class A>
+__global__ void __launch_bounds__(64) group_replay_kernel(A* a, const gal_group_op* ops, uint64_t n,
+                                                          uint64_t* results, uint32_t* status) {
+  const uint32_t lane = wave::lane_id();
+  char* base = a->table.memory;
+  for (uint64_t i = 0; i < n; i++) {
+    const gal_group_op* op = &ops[i];
+    const uint64_t lanes = op->lanes;
+    const uint32_t kind = op->kind;
+    bool bad = false;
+    if ((lanes >> lane) & 1ull) {
+      const uint64_t arg = op->arg[lane];
+      if (kind == GAL_OP_MALLOC) {
+        void* p = a->template malloc_at<false>(arg, op->tid[lane], op->slot);
+        results[i * 64 + lane] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
+      } else {
+        const uint64_t r = results[arg];
+        bad = r == GAL_NULL_OFFSET;
+        if (!bad) {
+          a->free(base + r);
+          results[i * 64 + lane] = 0;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (wave::ballot(bad)) {
+      if (lane == 0) *status = 1u;
+      return;
+    }
+  }
+}
+
+// Single-lane replay through the global allocator: global_malloc uses the
+// lane's hardware CU id and thread id 0.
+__global__ void global_replay_kernel(const gal_op* ops, uint64_t n, uint64_t* results, uint32_t* status,
+                                     uint32_t* cu_id) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  *cu_id = __smid();
+  char* base = ga::global_gallatin->table.memory;
+  for (uint64_t i = 0; i < n; i++) {
+    const gal_op op = ops[i];
+    if (op.kind == GAL_OP_MALLOC) {
+      void* p = ga::global_malloc(op.arg);
+      results[i] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
+    } else {
+      const uint64_t r = results[op.arg];
+      if (r == GAL_NULL_OFFSET) {
+        *status = 1u;
+        return;
+      }
+      ga::global_free(base + r);
       results[i] = 0;
     }
   }
@@ -359,16 +428,18 @@ __global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
   if (lane == 0 && bad) atom::add(&stripe_of(c)->tag_errors, (unsigned long long)bad);
 }
 
-__global__ void global_smoke_kernel(uint64_t n, uint64_t size, run_counters* c) {
+// the reference's global test kernel shape (tests/src/global_test.cu:34-110):
+// global_malloc, tag, check, global_free
+__global__ void global_plain_kernel(uint64_t n, uint64_t size, uint64_t seed, uint32_t verify, run_counters* c) {
   const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
   bool got = false, bad = false, miss = false;
   if (tid < n) {
     uint64_t* p = (uint64_t*)ga::global_malloc(size);
     if (p) {
       got = true;
-      const uint64_t tag = gw::payload_tag(7, tid);
-      bad = !tag_in(p, tag, GAL_VERIFY_ATOMIC);
-      bad |= !tag_out(p, tag, GAL_VERIFY_ATOMIC);
+      const uint64_t tag = gw::payload_tag(seed, tid);
+      bad = !tag_in(p, tag, verify);
+      bad |= !tag_out(p, tag, verify);
       ga::global_free(p);
     } else {
       miss = true;
@@ -619,6 +690,20 @@ struct impl {
     return hipGetLastError() == hipSuccess ? GAL_OK : GAL_ERR_HIP;
   }
 
+  static int group_replay(gal_allocator* h, const gal_group_op* d_ops, uint64_t n, uint64_t* d_results,
+                          uint32_t* d_status) {
+    group_replay_kernel<A><<<1, 64, 0, h->stream>>>((A*)h->dev, d_ops, n, d_results, d_status);
+    return hipGetLastError() == hipSuccess ? GAL_OK : GAL_ERR_HIP;
+  }
+  // the path malloc takes for a request (Gallatin::malloc_at's routing)
+  static uint32_t path_of(uint64_t size) {
+    const int bits = gu::get_first_bit_bigger(size < M ? M : size);
+    if (bits - A::kSmallestBits < (int)A::kNumTrees) return 0u;
+    const int block_tree = bits - A::kSmallestTreeBits;
+    if (block_tree < 0) return 0u;
+    return block_tree < (int)A::kNumTrees ? 1u : 2u;
+  }
+
   static void launch_plain(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify) {
     plain_kernel<A><<<blocks_for(n, kClaimBlock), kClaimBlock, 0, h->stream>>>((A*)h->dev, n, size, seed, verify, h->d_counters);
   }
@@ -803,7 +888,7 @@ struct impl {
   static constexpr impl_ops ops() {
     return impl_ops{S,         M,          B,       &create,       &create_host,       &allocated_size,    &destroy,
                     &describe,
-                    &snapshot, &replay,    &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print, &fill,
+                    &snapshot, &replay,    &group_replay, &path_of, &launch_plain, &launch_mixed_malloc, &launch_mixed_free, &print, &fill,
                     &batch_bin, &batch_pass, &batch_per_thread, &launch_churn, sizeof(table_t), &slab_init,
                     &slab_launch, &segment_words, &serve_headroom, &serve_resident_blocks, &serve_launch,
                     &serve_bin};
@@ -924,24 +1009,28 @@ int gal_supported(uint64_t* out, int cap) {
   return n;
 }
 
+// On failure the handle's own resources are released; the device allocator
+// `dev` stays with the caller, which destroys it.
 static int make_handle(const impl_ops* ops, void* dev, gal_allocator** out) {
   gal_allocator* h = new gal_allocator();
   h->ops = ops;
   h->dev = dev;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete h;
-    return GAL_ERR_HIP;
-  }
+  auto fail = [&](int rc) {
+    h->dev = nullptr;
+    gal_destroy(h);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(GAL_ERR_HIP);
   for (auto& e : h->ev)
-    if (hipEventCreate(&e) != hipSuccess) return GAL_ERR_HIP;
-  if (hipMalloc(&h->d_counters, sizeof(run_counters)) != hipSuccess) return GAL_ERR_HIP;
+    if (hipEventCreate(&e) != hipSuccess) return fail(GAL_ERR_HIP);
+  if (hipMalloc(&h->d_counters, sizeof(run_counters)) != hipSuccess) return fail(GAL_ERR_OOM);
   int khz = 0;
   int device = 0;
   if (hipGetDevice(&device) == hipSuccess &&
       hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
     h->tick_ns = 1e6 / (double)khz;
   int rc = ops->describe(h);
-  if (rc != GAL_OK) return rc;
+  if (rc != GAL_OK) return fail(rc);
   *out = h;
   return GAL_OK;
 }
@@ -967,7 +1056,8 @@ int gal_create(uint64_t segment_size, uint64_t min_size, uint64_t max_size, uint
 
 int gal_destroy(gal_allocator* h) {
   if (!h) return GAL_ERR_INVALID;
-  (void)hipStreamSynchronize(h->stream);
+  if (h->children > 0) return GAL_ERR_STATE;  // a batch, serving loop or slab table still uses it
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->dev) h->ops->destroy(h->dev);
   if (h->d_live) (void)hipFree(h->d_live);
   if (h->d_lat) (void)hipFree(h->d_lat);
@@ -1057,6 +1147,61 @@ int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* result
   if (rc == GAL_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = GAL_ERR_HIP;
   uint32_t status = 0;
   if (rc == GAL_OK && (hipMemcpy(results, d_res, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                       hipMemcpy(&status, d_status, 4, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = GAL_ERR_HIP;
+  if (rc == GAL_OK && status != 0) rc = GAL_ERR_INVALID;  // freed a failed malloc
+  if (d_ops) (void)hipFree(d_ops);
+  if (d_res) (void)hipFree(d_res);
+  if (d_status) (void)hipFree(d_status);
+  return rc;
+}
+
+int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint64_t* results) {
+  if (!h || (n && (!ops || !results))) return GAL_ERR_INVALID;
+  if (n == 0) return GAL_OK;
+  // Every op is checked before anything runs: a free names a lane of an
+  // earlier malloc op, each result at most once; a malloc op's sizes take
+  // one kind of path; a free op of several lanes frees no segment run.
+  std::vector<uint8_t> freed(n * 64, 0);
+  for (uint64_t i = 0; i < n; i++) {
+    const gal_group_op& op = ops[i];
+    if (op.lanes == 0) return GAL_ERR_INVALID;
+    if (op.kind == GAL_OP_MALLOC) {
+      bool large = false, small = false;
+      for (uint64_t m = op.lanes; m; m &= m - 1) {
+        const uint64_t size = op.arg[__builtin_ctzll(m)];
+        if (size == 0) return GAL_ERR_INVALID;
+        (h->ops->path_of(size) == 0 ? small : large) = true;
+      }
+      if (large && small) return GAL_ERR_INVALID;
+    } else if (op.kind == GAL_OP_FREE) {
+      const bool several = (op.lanes & (op.lanes - 1)) != 0;
+      for (uint64_t m = op.lanes; m; m &= m - 1) {
+        const uint64_t j = op.arg[__builtin_ctzll(m)];
+        if (j >= i * 64 || freed[j]) return GAL_ERR_INVALID;
+        const gal_group_op& src = ops[j / 64];
+        if (src.kind != GAL_OP_MALLOC || !((src.lanes >> (j % 64)) & 1ull)) return GAL_ERR_INVALID;
+        if (several && h->ops->path_of(src.arg[j % 64]) == 2) return GAL_ERR_INVALID;
+        freed[j] = 1;
+      }
+    } else {
+      return GAL_ERR_INVALID;
+    }
+  }
+  gal_group_op* d_ops = nullptr;
+  uint64_t* d_res = nullptr;
+  uint32_t* d_status = nullptr;
+  int rc = GAL_OK;
+  if (hipMalloc(&d_ops, n * sizeof(gal_group_op)) != hipSuccess || hipMalloc(&d_res, n * 64 * 8) != hipSuccess ||
+      hipMalloc(&d_status, 4) != hipSuccess)
+    rc = GAL_ERR_OOM;
+  if (rc == GAL_OK && (hipMemcpy(d_ops, ops, n * sizeof(gal_group_op), hipMemcpyHostToDevice) != hipSuccess ||
+                       hipMemset(d_res, 0xFF, n * 64 * 8) != hipSuccess || hipMemset(d_status, 0, 4) != hipSuccess))
+    rc = GAL_ERR_HIP;
+  if (rc == GAL_OK) rc = h->ops->group_replay(h, d_ops, n, d_res, d_status);
+  if (rc == GAL_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = GAL_ERR_HIP;
+  uint32_t status = 0;
+  if (rc == GAL_OK && (hipMemcpy(results, d_res, n * 64 * 8, hipMemcpyDeviceToHost) != hipSuccess ||
                        hipMemcpy(&status, d_status, 4, hipMemcpyDeviceToHost) != hipSuccess))
     rc = GAL_ERR_HIP;
   if (rc == GAL_OK && status != 0) rc = GAL_ERR_INVALID;  // freed a failed malloc
@@ -1211,7 +1356,8 @@ int gal_latency_samples(gal_allocator* h, uint32_t* ticks, uint64_t cap, uint64_
 
 int gal_read_pool(gal_allocator* h, uint64_t off, void* dst, uint64_t n) {
   if (!h || !dst) return GAL_ERR_INVALID;
-  if (off + n > h->info.num_segments * h->info.segment_size) return GAL_ERR_INVALID;
+  const uint64_t pool = h->info.num_segments * h->info.segment_size;
+  if (off > pool || n > pool - off) return GAL_ERR_INVALID;
   if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
   return hipMemcpy(dst, (const char*)h->info.pool_device_ptr + off, n, hipMemcpyDeviceToHost) == hipSuccess
              ? GAL_OK
@@ -1233,6 +1379,7 @@ int gal_batch_destroy(gal_batch* b) {
   void* bufs[] = {b->d_sizes, b->d_perm, b->d_counts, b->d_offsets, b->d_totals, b->d_ptrs};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  if (b->h) b->h->children--;
   delete b;
   return GAL_OK;
 }
@@ -1242,6 +1389,7 @@ int gal_batch_create(gal_allocator* h, uint64_t capacity, gal_batch** out) {
   *out = nullptr;
   gal_batch* b = new gal_batch();
   b->h = h;
+  h->children++;
   b->capacity = capacity;
   const uint64_t waves = (capacity + kBinWaveRequests - 1) / kBinWaveRequests;
   const bool ok = hipMalloc(&b->d_sizes, capacity * 8) == hipSuccess &&
@@ -1414,6 +1562,7 @@ int gal_slab_create(gal_allocator* h, uint64_t buckets, uint64_t seed, gal_slab_
     delete t;
     return GAL_ERR_OOM;
   }
+  h->children++;
   h->ops->slab_init(t, seed);
   *out = t;
   return GAL_OK;
@@ -1477,6 +1626,7 @@ int gal_slab_destroy(gal_slab_table* t, uint64_t* slabs_freed) {
   (void)hipFree(t->dev);
   (void)hipFree(t->heads);
   (void)hipFree(t->counters);
+  t->h->children--;
   delete t;
   return rc;
 }
@@ -1528,6 +1678,7 @@ int gal_serve_destroy(gal_serve* s) {
   void* bufs[] = {s->d_reqs, s->d_checksums, s->d_admit, s->d_done, s->d_state, s->d_hist};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  if (s->h) s->h->children--;
   delete s;
   return GAL_OK;
 }
@@ -1537,6 +1688,7 @@ int gal_serve_create(gal_allocator* h, uint64_t capacity, gal_serve** out) {
   *out = nullptr;
   gal_serve* s = new gal_serve();
   s->h = h;
+  h->children++;
   s->capacity = capacity;
   const bool ok = hipMalloc(&s->d_reqs, capacity * sizeof(serve_req)) == hipSuccess &&
                   hipMalloc(&s->d_checksums, capacity * 8) == hipSuccess &&
@@ -1877,6 +2029,7 @@ int gal_init_global_allocator(uint64_t num_bytes, uint64_t seed, int print_info)
 
 int gal_free_global_allocator(void) {
   if (!g_global) return GAL_ERR_INVALID;
+  if (g_global->children > 0) return GAL_ERR_STATE;
   ga::free_global_allocator();
   g_global->dev = nullptr;  // already released
   gal_destroy(g_global);
@@ -1906,6 +2059,7 @@ int gal_init_global_allocator_host(uint64_t num_bytes, uint64_t seed, int print_
 
 int gal_free_global_allocator_host(void) {
   if (!g_global_host) return GAL_ERR_INVALID;
+  if (g_global_host->children > 0) return GAL_ERR_STATE;
   ga::free_global_allocator_host();
   g_global_host->dev = nullptr;
   gal_destroy(g_global_host);
@@ -1929,6 +2083,7 @@ int gal_init_global_allocator_combined(uint64_t num_bytes, uint64_t host_bytes, 
 }
 
 int gal_free_global_allocator_combined(void) {
+  if ((g_global && g_global->children > 0) || (g_global_host && g_global_host->children > 0)) return GAL_ERR_STATE;
   const int a = gal_free_global_allocator();
   const int b = gal_free_global_allocator_host();
   return a != GAL_OK ? a : b;
@@ -2000,18 +2155,59 @@ int gal_global_poison(uint64_t size, int32_t* codes, uint32_t cap) {
   return rc;
 }
 
-int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors) {
-  if (!g_global || n_threads == 0) return GAL_ERR_INVALID;
-  gal_run_result r;
-  int rc = timed_run(g_global, &r, [&](launch_timer& t) {
+int gal_global_plain(uint64_t n_threads, uint64_t size, uint64_t seed, uint32_t verify, gal_run_result* out) {
+  if (!g_global || n_threads == 0 || size == 0 || verify > GAL_VERIFY_ATOMIC) return GAL_ERR_INVALID;
+  return timed_run(g_global, out, [&](launch_timer& t) {
     if (!t.begin()) return GAL_ERR_INVALID;
-    global_smoke_kernel<<<blocks_for(n_threads, 256), 256, 0, g_global->stream>>>(n_threads, size,
-                                                                                   g_global->d_counters);
+    global_plain_kernel<<<blocks_for(n_threads, kClaimBlock), kClaimBlock, 0, g_global->stream>>>(
+        n_threads, size, seed, verify, g_global->d_counters);
     t.end(0);
     return GAL_OK;
   });
+}
+
+int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors) {
+  gal_run_result r;
+  std::memset(&r, 0, sizeof(r));
+  const int rc = gal_global_plain(n_threads, size, 7, GAL_VERIFY_ATOMIC, &r);
   if (misses) *misses = r.misses;
   if (tag_errors) *tag_errors = r.tag_errors;
+  return rc;
+}
+
+int gal_global_replay(const gal_op* ops, uint64_t n, uint64_t* results, uint32_t* cu_id) {
+  if (!g_global || (n && (!ops || !results))) return GAL_ERR_INVALID;
+  if (n == 0) return GAL_OK;
+  for (uint64_t i = 0; i < n; i++) {
+    if (ops[i].kind == GAL_OP_FREE) {
+      if (ops[i].arg >= i || ops[ops[i].arg].kind != GAL_OP_MALLOC) return GAL_ERR_INVALID;
+    } else if (ops[i].kind != GAL_OP_MALLOC || ops[i].arg == 0) {
+      return GAL_ERR_INVALID;
+    }
+  }
+  gal_op* d_ops = nullptr;
+  uint64_t* d_res = nullptr;
+  uint32_t* d_word = nullptr;  // status, CU id
+  int rc = GAL_OK;
+  if (hipMalloc(&d_ops, n * sizeof(gal_op)) != hipSuccess || hipMalloc(&d_res, n * 8) != hipSuccess ||
+      hipMalloc(&d_word, 8) != hipSuccess)
+    rc = GAL_ERR_OOM;
+  if (rc == GAL_OK && (hipMemcpy(d_ops, ops, n * sizeof(gal_op), hipMemcpyHostToDevice) != hipSuccess ||
+                       hipMemset(d_word, 0, 8) != hipSuccess))
+    rc = GAL_ERR_HIP;
+  if (rc == GAL_OK) {
+    global_replay_kernel<<<1, 1, 0, g_global->stream>>>(d_ops, n, d_res, d_word, d_word + 1);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_global->stream) != hipSuccess) rc = GAL_ERR_HIP;
+  }
+  uint32_t words[2] = {0, 0};
+  if (rc == GAL_OK && (hipMemcpy(results, d_res, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                       hipMemcpy(words, d_word, 8, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = GAL_ERR_HIP;
+  if (rc == GAL_OK && words[0] != 0) rc = GAL_ERR_INVALID;  // freed a failed malloc
+  if (rc == GAL_OK && cu_id) *cu_id = words[1];
+  if (d_ops) (void)hipFree(d_ops);
+  if (d_res) (void)hipFree(d_res);
+  if (d_word) (void)hipFree(d_word);
   return rc;
 }
 

@@ -403,7 +403,10 @@ struct Gallatin {
   // `subgroup` splits a class's lanes further (lanes with equal class and
   // subgroup form one group); malloc() passes 0, so a group is every lane of
   // the class. `allocs` is the number of allocations a lane's `count` slices
-  // stand for: 1 for malloc, the run length for a bulk claim.
+  // stand for: 1 for malloc, the run length for a bulk claim. `spread`
+  // false keeps every group on the reference's pinned-slot walk (no start at
+  // an overflow slot), for deterministic multi-lane replays.
+  template <bool spread = true>
   __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint32_t allocs,
                                              uint64_t tid, uint32_t hint, uint32_t subgroup, uint32_t attempt,
                                              uint64_t& result, bool& done, bool& oom, bool& waited) {
@@ -421,7 +424,7 @@ struct Gallatin {
     // always for a single lane) the reference's walk over the pinned slots.
     bool placed = false;
     bool dry = false;  // the leader found the class out of memory in this pass
-    if (attempt == 0 && usable > pinned_slots_of(tree) && (looking & (looking - 1)) != 0) {
+    if (spread && attempt == 0 && usable > pinned_slots_of(tree) && (looking & (looking - 1)) != 0) {
       const uint32_t si = lead_hint % usable;
       // read with the slot, in the same round trip: the class's tree summary
       // and the free-segment gauge
@@ -576,6 +579,7 @@ struct Gallatin {
   // structurizer runs a loop's exit block once, after the loop, for all lanes
   // that left, so per-group work must not sit in one. `oom` is set when the
   // failure is a confirmed out-of-memory, which a further call would repeat.
+  template <bool spread = true>
   __device__ __forceinline__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint32_t allocs,
                                                               uint64_t tid, uint32_t hint, uint32_t subgroup,
                                                               bool& oom) {
@@ -590,7 +594,8 @@ struct Gallatin {
     for (int pass = 0; pass < kBusyPasses && looking; pass++) {
       bool waited = false;
       if ((looking >> wave::lane_id()) & 1ull)
-        claim_pass(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)pass, result, done, oom, waited);
+        claim_pass<spread>(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)pass, result, done, oom,
+                           waited);
       if (!done && !waited) tries++;
       looking &= ~wave::ballot(done || tries >= (uint32_t)kSliceAttempts);
     }
@@ -702,7 +707,8 @@ struct Gallatin {
   // malloc with the slot selector and hash id given explicitly; malloc()
   // passes the hardware CU id and the global thread id. `subgroup` (0 for
   // malloc) splits the lanes of one class into separate claim groups: the
-  // coalescing width of the batched kernels.
+  // coalescing width of the batched kernels. `spread` as in claim_pass.
+  template <bool spread = true>
   __device__ __forceinline__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint, uint32_t subgroup = 0) {
     uint32_t tree = get_tree_id_from_size(size);
     uint32_t count = 1;
@@ -715,7 +721,7 @@ struct Gallatin {
     uint64_t offset = utils::kFail;
     bool oom = false;
     for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail && !oom; attempt++)
-      offset = malloc_slice_allocation(tree, count, 1u, tid, hint, subgroup, oom);
+      offset = malloc_slice_allocation<spread>(tree, count, 1u, tid, hint, subgroup, oom);
     if (offset == utils::kFail) return nullptr;
     return offset_to_allocation(offset, tree);
   }
@@ -795,9 +801,14 @@ struct Gallatin {
 
   // A run of segments (reference gallatin.cuh:1283-1302): release the id
   // first so that no claimer can find the segment in the tree while the old
-  // id is still set.
+  // id is still set. Only the lane whose reset succeeds returns the segments
+  // and adds them to the free-segment gauge: a second free of the same run
+  // (two lanes at once, or a double free) would otherwise count it twice.
   __device__ __noinline__ void free_segment_run(uint64_t segment, uint32_t tree) {
-    if (!table.reset_tree_id(segment, tree)) count_error(kErrRunReset);
+    if (!table.reset_tree_id(segment, tree)) {
+      count_error(kErrRunReset);
+      return;
+    }
     atom::fence_release();
     segment_tree.return_multiple(segment, tree - kNumTrees - 1);
     atom::add(ctl_word(control, kCtlFreeSegs), tree - kNumTrees - 1);

@@ -143,6 +143,38 @@ int gal_snapshot(gal_allocator* h, void* buf, uint64_t cap, uint64_t* needed);
 /* Single-lane deterministic replay; results[i] per op (see above). */
 int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* results);
 
+/* ---- multi-lane deterministic replay ----
+ * One wavefront runs the ops in order, nothing else runs on the allocator.
+ * Op i is one call site of malloc (or free) reached by the lanes of `lanes`
+ * together, with exactly that exec mask, the way the lanes of a warp that
+ * are active at a call in a reference kernel enter Gallatin::malloc
+ * together (gallatin.cuh:1186-1276, teams by class at :881-885, one claim
+ * per team at block.cuh:245-281; frees grouped by block at :1774-1827).
+ * The claims use the reference's pinned-slot walk from `slot` for every
+ * group (the overflow-slot spreading that concurrent claims use is off), so
+ * the results are a function of the op stream alone and the CPU oracle
+ * reproduces them bit for bit.
+ *   GAL_OP_MALLOC: arg[l] = request size of lane l. The lanes' sizes are all
+ *                  on the slice path (up to min_size * 2048 bytes) or all
+ *                  above it (the block and segment requests are served one
+ *                  lane at a time in lane order).
+ *   GAL_OP_FREE:   arg[l] = index j * 64 + k of the result freed (lane k of an
+ *                  earlier malloc op j, which must have succeeded and not
+ *                  been freed). A free op of several lanes frees slices or
+ *                  whole blocks only; a run of segments is freed alone.
+ * results[i * 64 + l]: lane l's byte offset from the pool base (malloc),
+ * 0 (free), GAL_NULL_OFFSET (a failed malloc, or a lane not in the op).
+ * No counterpart in the reference (whose concurrent results are not
+ * reproducible); it pins the group claim semantics against the oracle. */
+typedef struct gal_group_op {
+  uint32_t kind;    /* GAL_OP_MALLOC or GAL_OP_FREE */
+  uint32_t slot;    /* CU id stand-in of the wavefront (the reference's %smid) */
+  uint64_t lanes;   /* lanes taking part, nonzero */
+  uint64_t arg[64]; /* per lane, see above */
+  uint64_t tid[64]; /* per lane: thread id fed to the hashed segment search */
+} gal_group_op;
+int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint64_t* results);
+
 /* Device error counters: internal inconsistencies (where the reference would
  * trap) and rejected frees of pointers the allocator does not own. */
 int gal_errors(gal_allocator* h, uint64_t* internal_errors, uint64_t* bad_frees);
@@ -173,6 +205,12 @@ int gal_free_global_allocator(void);
 int gal_print_global_stats(void);
 /* one kernel of n threads: global_malloc(size), tag, verify, global_free */
 int gal_global_smoke(uint64_t n_threads, uint64_t size, uint64_t* misses, uint64_t* tag_errors);
+/* Single-lane replay through global_malloc / global_free (reference
+ * global_allocator.cuh:72-82): one lane of one kernel runs the ops in order
+ * (gal_op; slot and tid are ignored: global_malloc uses the lane's hardware
+ * CU id and its thread id 0). *cu_id reports the CU id the lane ran on, the
+ * slot the same stream replays with through gal_replay or the oracle. */
+int gal_global_replay(const gal_op* ops, uint64_t n, uint64_t* results, uint32_t* cu_id);
 
 /* Global allocator over mapped host memory, and both together. Replace
  * init_global_allocator_host / free_global_allocator_host /
@@ -227,6 +265,13 @@ typedef struct gal_run_result {
  * (tests/src/gallatin_test.cu:58-110, tests/src/global_churn.cu:185-245). */
 int gal_run_plain(gal_allocator* h, uint64_t n_threads, uint64_t size, uint64_t seed, uint32_t verify,
                   gal_run_result* out);
+/* The same kernel through the global allocator (gal_init_global_allocator),
+ * timed on its handle's stream: n_threads threads each global_malloc(size),
+ * write a tag drawn from (seed, thread), check it (GAL_VERIFY_*) and
+ * global_free. The reference's global test kernel
+ * (tests/src/global_test.cu:34-110) through global_malloc/global_free,
+ * reference global_allocator.cuh:72-82. */
+int gal_global_plain(uint64_t n_threads, uint64_t size, uint64_t seed, uint32_t verify, gal_run_result* out);
 
 /* Mixed-size rounds. Round r: thread i mallocs 16 << (k mod 9) bytes with k
  * drawn from (seed, r, i), stores the pointer in the live table and tags the

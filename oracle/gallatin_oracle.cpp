@@ -499,6 +499,16 @@ struct Table {
   }
 };
 
+// gallatin.cuh:1646: re-list a segment in its class tree when the slot a
+// returned block takes crosses a tenth of the segment: 1.0*slot/nb >= .1 &&
+// 1.0*(slot-1)/nb < .1 (REREGISTER_CUTOFF, :123). nb is a power of two, so
+// no slot/nb equals 0.1 and the test is exactly 10*slot > nb &&
+// 10*(slot-1) < nb; slot == 0 makes the unsigned slot-1 wrap, and the second
+// clause is false.
+static inline bool reregister_now(uint32_t slot, uint64_t nb) {
+  return slot >= 1 && 10ull * slot > nb && 10ull * (slot - 1) < nb;
+}
+
 // ---------------------------------------------------------------- allocator
 struct Alloc {
   uint64_t SEG = 0, MIN = 0, MAX = 0, seed = 0, pool_bytes = 0;
@@ -589,10 +599,7 @@ struct Alloc {
   void return_block(uint64_t block, uint64_t segment, uint32_t tree) {
     uint64_t nb = table.blocks_per_segment(tree);
     uint32_t reserved_slot = table.reserve_segment_slot(block, segment, nb, faults);
-    // 1.0*slot/nb >= .1 && 1.0*(slot-1)/nb < .1 (REREGISTER_CUTOFF, :123, :1646):
-    // nb is a power of two, so no slot/nb equals 0.1 and the test is exactly
-    // 10*slot > nb && 10*(slot-1) < nb; slot == 0 makes (slot-1) wrap, false.
-    if (reserved_slot >= 1 && 10ull * reserved_slot > nb && 10ull * (reserved_slot - 1) < nb) {
+    if (reregister_now(reserved_slot, nb)) {
       sub_trees[tree].insert_force_update(segment);
     }
     bool need_to_deregister = table.finish_freeing_block(segment, nb);
@@ -784,6 +791,242 @@ struct Alloc {
     return GAL_OK;
   }
 
+  // ---- one warp's call site executed by several lanes together.
+  //
+  // The reference forms its teams with cg::coalesced_threads() and
+  // labeled_partition by size class (gallatin.cuh:881-885): every active lane
+  // asking for one class is in one team, whose thread ranks follow lane
+  // order. Per team and attempt: rank 0 picks the pinned slot
+  // (get_valid_block_index, with the warp's smid) and shares the block
+  // (:887-894); an exclusive scan of the lanes' slice counts in rank order
+  // (:982); the team's last rank adds the team total to the block's malloc
+  // counter and broadcasts the old value (block.cuh:245-264); each lane's
+  // allocation is old count + its prefix (block.cuh:373-388); the leftovers
+  // of fully valid multi-slice lanes (count - 1) and of the lane that
+  // straddles slice 4095 (4096 - allocation) are summed and added to the free
+  // counter once (block.cuh:267-281); if any lane's range covers slice 4095,
+  // rank 0 replaces the block (:1015-1026). Lanes that got no slice loop
+  // again and re-form teams among themselves.
+  //
+  // Teams of different classes touch different slots and blocks, so their
+  // claims commute; their rare work (refill, replace, recycle) changes the
+  // shared trees and runs team by team in the order of the teams' first
+  // lanes. The lanes of one call site run in lockstep, so a lane's attempt
+  // budget (10 calls of up to 500 passes, :1263-1268, :878) advances with
+  // every other lane still looking. A team that finds its slot empty refills
+  // it (Q6) and claims in the next pass; one whose refill finds no memory
+  // fails for all its lanes.
+
+  struct Team {
+    uint64_t lanes = 0;
+    uint32_t tree = 0;
+    int index = 0;
+    uint32_t entry = 0;
+    uint64_t block = 0;
+    uint32_t c0 = 0;
+    bool empty = false, replace = false, completed = false, stale = false;
+    uint32_t give_back = 0;
+  };
+
+  // one pass of the slice claim over the lanes of `looking`
+  void team_pass(uint64_t looking, const uint32_t* tree, const uint32_t* count, const uint64_t* tid, uint32_t slot,
+                 uint64_t* result, bool* done, bool* oom) {
+    std::vector<Team> teams;
+    for (uint64_t todo = looking; todo;) {
+      const int first = ffs64(todo) - 1;
+      Team t;
+      t.tree = tree[first];
+      for (uint64_t m = todo; m; m &= m - 1) {
+        const int l = ffs64(m) - 1;
+        if (tree[l] == t.tree) t.lanes |= 1ull << l;
+      }
+      todo &= ~t.lanes;
+      teams.push_back(t);
+    }
+    // claims: each team on its own class's slot and block
+    for (Team& t : teams) {
+      t.index = valid_block_index(t.tree, slot);
+      t.entry = slots[t.tree][t.index];
+      t.empty = t.entry == 0u;
+      if (t.empty) continue;
+      t.block = t.entry - 1;
+      Block& b = table.blocks[t.block];
+      uint32_t total = 0;
+      for (uint64_t m = t.lanes; m; m &= m - 1) total += count[ffs64(m) - 1];
+      const uint32_t merged = b.malloc_counter;
+      b.malloc_counter = merged + total;
+      t.c0 = clip_count(merged);
+      t.stale = !check_valid(merged, t.tree);
+      uint32_t prefix = 0, held = 0;
+      bool crosses = false;
+      for (uint64_t m = t.lanes; m; m &= m - 1) {
+        const int l = ffs64(m) - 1;
+        const uint64_t allocation = (uint64_t)t.c0 + prefix;
+        crosses |= allocation <= 4095 && allocation + count[l] > 4095;
+        if (allocation + count[l] <= 4096) {
+          held++;
+          if (!t.stale) {
+            result[l] = allocation + t.block * 4096;
+            done[l] = true;
+          }
+        }
+        prefix += count[l];
+      }
+      const uint32_t in_range = t.c0 >= 4096 ? 0u : std::min<uint32_t>(t.c0 + total, 4096u) - t.c0;
+      const uint32_t written_off = in_range - held;
+      if (written_off > 0) {
+        const uint32_t old = b.free_counter;
+        b.free_counter = old + written_off;
+        t.completed = old + written_off == 4096;
+      }
+      t.give_back = t.stale ? held : 0u;
+      t.replace = crosses || t.c0 > 4095 || t.stale;
+    }
+    // rare work, team by team in the order of their first lanes
+    for (Team& t : teams) {
+      const uint64_t leader_tid = tid[ffs64(t.lanes) - 1];
+      if (t.empty) {
+        int64_t fresh = request_new_block_from_tree(t.tree, leader_tid);
+        if (fresh >= 0 && slots[t.tree][t.index] == 0u) {
+          slots[t.tree][t.index] = (uint32_t)(fresh + 1);
+        } else {
+          if (fresh >= 0) free_block((uint64_t)fresh);
+          for (uint64_t m = t.lanes; m; m &= m - 1) {
+            const int l = ffs64(m) - 1;
+            done[l] = true;
+            oom[l] = true;
+            result[l] = FAIL;
+          }
+        }
+        continue;
+      }
+      if (t.replace) replace_block(t.tree, t.index, t.block, leader_tid);
+      if (t.completed) {  // Q13
+        table.blocks[t.block].free_counter = 0;
+        free_block(t.block);
+      }
+      if (t.give_back > 0) {
+        Block& b = table.blocks[t.block];
+        const uint32_t old = b.free_counter;
+        b.free_counter = old + t.give_back;
+        if (old + t.give_back == 4096) {
+          b.free_counter = 0;
+          free_block(t.block);
+        }
+      }
+    }
+  }
+
+  // malloc called by the lanes of `lanes` together. Requests above the slice
+  // range are served one lane at a time in lane order (they take no team
+  // claim); a call site mixing both kinds is not modelled (the order of the
+  // two branches is up to the compiler).
+  int group_malloc(uint64_t lanes, const uint64_t* size, const uint64_t* tid, uint32_t slot, uint64_t* out) {
+    uint32_t tree[64] = {}, count[64] = {};
+    bool large = false, small = false;
+    for (uint64_t m = lanes; m; m &= m - 1) {
+      const int l = ffs64(m) - 1;
+      uint32_t t = tree_id_from_size(size[l]);
+      uint32_t c = 1;
+      if (t >= num_trees) {
+        if (first_bit_bigger(size[l]) - first_bit_bigger(MIN * 4096) >= 0) {
+          large = true;
+          continue;
+        }
+        c = (uint32_t)(1ull << (t - (num_trees - 1)));
+        t = num_trees - 1;
+      }
+      small = true;
+      tree[l] = t;
+      count[l] = c;
+    }
+    if (large && small) return GAL_ERR_INVALID;
+    if (large) {
+      for (uint64_t m = lanes; m; m &= m - 1) {
+        const int l = ffs64(m) - 1;
+        out[l] = malloc(size[l], tid[l], slot);
+      }
+      return GAL_OK;
+    }
+    uint64_t result[64];
+    bool done[64], oom[64];
+    uint32_t calls[64] = {};
+    for (int l = 0; l < 64; l++) {
+      result[l] = FAIL;
+      oom[l] = false;
+    }
+    // malloc's attempt loop (gallatin.cuh:1263-1268): lanes still without an
+    // allocation and not out of memory call the slice claim again together
+    for (;;) {
+      uint64_t calling = 0;
+      for (uint64_t m = lanes; m; m &= m - 1) {
+        const int l = ffs64(m) - 1;
+        if (result[l] == FAIL && !oom[l] && calls[l] < 10) calling |= 1ull << l;
+      }
+      if (!calling) break;
+      uint32_t tries[64] = {};
+      for (int l = 0; l < 64; l++) done[l] = false;
+      for (uint64_t m = calling; m; m &= m - 1) calls[ffs64(m) - 1]++;
+      uint64_t looking = calling;
+      while (looking) {
+        team_pass(looking, tree, count, tid, slot, result, done, oom);
+        uint64_t still = 0;
+        for (uint64_t m = looking; m; m &= m - 1) {
+          const int l = ffs64(m) - 1;
+          if (!done[l]) tries[l]++;
+          if (!done[l] && tries[l] < 500) still |= 1ull << l;
+        }
+        looking = still;
+      }
+    }
+    for (uint64_t m = lanes; m; m &= m - 1) {
+      const int l = ffs64(m) - 1;
+      out[l] = result[l] == FAIL ? FAIL : offset_to_bytes(result[l], tree[l]);
+    }
+    return GAL_OK;
+  }
+
+  // free called by the lanes of `lanes` together on slice or whole-block
+  // allocations (reference gallatin.cuh:1774-1827): lanes are grouped by
+  // block (labeled_partition by block id), each group's rank 0 adds the
+  // group size to the block's free counter, and the blocks whose count
+  // completes are returned group by group in lane order.
+  int group_free(uint64_t lanes, const uint64_t* byte_offset) {
+    uint64_t block_of[64] = {};
+    for (uint64_t m = lanes; m; m &= m - 1) {
+      const int l = ffs64(m) - 1;
+      const uint64_t off = byte_offset[l];
+      if (off >= table.num_segments * SEG) return GAL_ERR_INVALID;
+      const uint64_t segment = off / SEG;
+      const uint32_t tree_id = table.chunk_ids[segment];
+      if (tree_id >= num_trees) return GAL_ERR_INVALID;
+      const uint64_t offset = (off - segment * SEG) / table.tree_alloc_size(tree_id) +
+                              segment * table.max_allocations_per_segment();
+      block_of[l] = offset / 4096;
+    }
+    std::vector<uint64_t> completed;
+    for (uint64_t todo = lanes; todo;) {
+      const uint64_t block = block_of[ffs64(todo) - 1];
+      uint32_t n = 0;
+      for (uint64_t m = todo; m; m &= m - 1) {
+        const int l = ffs64(m) - 1;
+        if (block_of[l] == block) {
+          n++;
+          todo &= ~(1ull << l);
+        }
+      }
+      Block& b = table.blocks[block];
+      const uint32_t old = b.free_counter;
+      b.free_counter = old + n;
+      if (old + n == 4096) completed.push_back(block);
+    }
+    for (uint64_t block : completed) {
+      table.blocks[block].free_counter = 0;
+      free_block(block);
+    }
+    return GAL_OK;
+  }
+
   // gallatin.cuh:726-787 boot_block
   bool boot_block(uint32_t tree, int slot) {
     int64_t block = request_new_block_from_tree(tree, 0);
@@ -898,6 +1141,7 @@ uint64_t orc_murmur64a(uint64_t key, uint64_t seed) { return orc::murmur64a(key,
 int orc_first_bit_bigger(uint64_t x) { return orc::first_bit_bigger(x); }
 int orc_cfcll(uint64_t bits) { return orc::cfcll(bits); }
 uint64_t orc_splitmix64(uint64_t x) { return orc::splitmix64(x); }
+int orc_reregister(uint32_t slot, uint64_t nb) { return orc::reregister_now(slot, nb); }
 
 int orc_route(uint64_t seg, uint64_t min, uint64_t max, uint64_t size, uint32_t* path, uint32_t* tree,
               uint64_t* count) {
@@ -1011,6 +1255,48 @@ int orc_replay(orc_alloc* h, const gal_op* ops, uint64_t n, uint64_t* results) {
     }
   }
   return GAL_OK;
+}
+
+int orc_group_replay(orc_alloc* h, const gal_group_op* ops, uint64_t n, uint64_t* results) {
+  if (!h || (n && (!ops || !results))) return GAL_ERR_INVALID;
+  for (uint64_t k = 0; k < n * 64; k++) results[k] = GAL_NULL_OFFSET;
+  for (uint64_t i = 0; i < n; i++) {
+    const gal_group_op& op = ops[i];
+    if (op.lanes == 0) return GAL_ERR_INVALID;
+    uint64_t* out = results + i * 64;
+    if (op.kind == GAL_OP_MALLOC) {
+      int rc = h->a.group_malloc(op.lanes, op.arg, op.tid, op.slot, out);
+      if (rc != GAL_OK) return rc;
+      continue;
+    }
+    if (op.kind != GAL_OP_FREE) return GAL_ERR_INVALID;
+    uint64_t offsets[64] = {};
+    for (uint64_t m = op.lanes; m; m &= m - 1) {
+      const int l = __builtin_ctzll(m);
+      const uint64_t j = op.arg[l];
+      if (j >= i * 64 || !((ops[j / 64].lanes >> (j % 64)) & 1ull) || ops[j / 64].kind != GAL_OP_MALLOC ||
+          results[j] == GAL_NULL_OFFSET)
+        return GAL_ERR_INVALID;
+      offsets[l] = results[j];
+    }
+    int rc = (op.lanes & (op.lanes - 1)) == 0 ? h->a.free(offsets[__builtin_ctzll(op.lanes)])
+                                               : h->a.group_free(op.lanes, offsets);
+    if (rc != GAL_OK) return rc;
+    for (uint64_t m = op.lanes; m; m &= m - 1) out[__builtin_ctzll(m)] = 0;
+  }
+  return GAL_OK;
+}
+
+int orc_group_malloc(orc_alloc* h, uint64_t lanes, const uint64_t* sizes, const uint64_t* tids, uint32_t slot,
+                     uint64_t* out) {
+  if (!h || !lanes || !sizes || !tids || !out) return GAL_ERR_INVALID;
+  return h->a.group_malloc(lanes, sizes, tids, slot, out);
+}
+
+int orc_group_free(orc_alloc* h, uint64_t lanes, const uint64_t* byte_offsets) {
+  if (!h || !lanes || !byte_offsets) return GAL_ERR_INVALID;
+  if ((lanes & (lanes - 1)) == 0) return h->a.free(byte_offsets[__builtin_ctzll(lanes)]);
+  return h->a.group_free(lanes, byte_offsets);
 }
 
 int orc_snapshot(orc_alloc* h, void* buf, uint64_t cap, uint64_t* needed) {

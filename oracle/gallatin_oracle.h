@@ -25,6 +25,8 @@ uint64_t orc_murmur64a(uint64_t key, uint64_t seed);
 int orc_first_bit_bigger(uint64_t x);
 int orc_cfcll(uint64_t bits);
 uint64_t orc_splitmix64(uint64_t x);
+/* the re-register test return_block applies to a returned block's queue slot */
+int orc_reregister(uint32_t slot, uint64_t nb);
 
 /* Size routing: path 0 = slice, 1 = block, 2 = segments, 3 = unsupported.
  * tree = class used, count = slices (slice path) or segments (segment path). */
@@ -58,6 +60,15 @@ uint64_t orc_malloc(orc_alloc* a, uint64_t size, uint64_t tid, uint32_t slot);
 /* frees the allocation at byte offset; returns GAL_OK or GAL_ERR_INVALID */
 int orc_free(orc_alloc* a, uint64_t byte_offset);
 int orc_replay(orc_alloc* a, const gal_op* ops, uint64_t n, uint64_t* results);
+/* Multi-lane replay with the semantics of gal_group_replay: op i is one warp
+ * call site reached by the lanes of ops[i].lanes together (teams by class,
+ * one claim per team, frees grouped by block); results[i * 64 + l]. */
+int orc_group_replay(orc_alloc* a, const gal_group_op* ops, uint64_t n, uint64_t* results);
+/* One such call site at a time: malloc by the lanes of `lanes` (sizes,
+ * tids and out indexed by lane), or free of the byte offsets of `lanes`. */
+int orc_group_malloc(orc_alloc* a, uint64_t lanes, const uint64_t* sizes, const uint64_t* tids, uint32_t slot,
+                     uint64_t* out);
+int orc_group_free(orc_alloc* a, uint64_t lanes, const uint64_t* byte_offsets);
 int orc_snapshot(orc_alloc* a, void* buf, uint64_t cap, uint64_t* needed);
 /* count of internal consistency faults seen (reference traps) */
 uint64_t orc_faults(orc_alloc* a);

@@ -24,11 +24,16 @@ class _Op(ctypes.Structure):
     _fields_ = [("kind", u32), ("slot", u32), ("arg", u64), ("tid", u64)]
 
 
+class _GroupOp(ctypes.Structure):
+    _fields_ = [("kind", u32), ("slot", u32), ("lanes", u64), ("arg", u64 * 64), ("tid", u64 * 64)]
+
+
 _SIG = {
     "orc_murmur64a": (u64, [u64, u64]),
     "orc_first_bit_bigger": (ctypes.c_int, [u64]),
     "orc_cfcll": (ctypes.c_int, [u64]),
     "orc_splitmix64": (u64, [u64]),
+    "orc_reregister": (ctypes.c_int, [u32, u64]),
     "orc_route": (ctypes.c_int, [u64, u64, u64, u64, ctypes.POINTER(u32), ctypes.POINTER(u32), pu64]),
     "orc_veb_create": (vp, [u64, u64, ctypes.c_int]),
     "orc_veb_destroy": (None, [vp]),
@@ -50,6 +55,9 @@ _SIG = {
     "orc_malloc": (u64, [vp, u64, u64, u32]),
     "orc_free": (ctypes.c_int, [vp, u64]),
     "orc_replay": (ctypes.c_int, [vp, ctypes.POINTER(_Op), u64, pu64]),
+    "orc_group_replay": (ctypes.c_int, [vp, ctypes.POINTER(_GroupOp), u64, pu64]),
+    "orc_group_malloc": (ctypes.c_int, [vp, u64, pu64, pu64, u32, pu64]),
+    "orc_group_free": (ctypes.c_int, [vp, u64, pu64]),
     "orc_snapshot": (ctypes.c_int, [vp, vp, u64, pu64]),
     "orc_faults": (u64, [vp]),
     "orc_bench_mixed": (ctypes.c_int, [u64, u64, u64, u32, pu64, pu64, ctypes.POINTER(ctypes.c_double)]),
@@ -179,6 +187,37 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_replay failed: {rc}")
         return np.frombuffer(out, dtype=np.uint64, count=len(ops)).copy()
+
+    def group_replay(self, ops):
+        """ops: (kind, slot, lanes, args, tids) per op; returns an (n, 64) array."""
+        ops = list(ops)
+        arr = (_GroupOp * max(1, len(ops)))()
+        for i, (kind, slot, lanes, args, tids) in enumerate(ops):
+            arr[i].kind, arr[i].slot, arr[i].lanes = kind, slot, lanes
+            for lane in range(64):
+                if (lanes >> lane) & 1:
+                    arr[i].arg[lane] = int(args[lane])
+                    arr[i].tid[lane] = int(tids[lane])
+        out = (u64 * max(1, 64 * len(ops)))()
+        rc = lib.orc_group_replay(self.h, arr, len(ops), out)
+        if rc != 0:
+            raise RuntimeError(f"orc_group_replay failed: {rc}")
+        return np.frombuffer(out, dtype=np.uint64, count=64 * len(ops)).copy().reshape(len(ops), 64)
+
+    def group_malloc(self, lanes, sizes, tids, slot):
+        """malloc by the lanes of `lanes` together; sizes/tids: 64 entries by lane."""
+        sz = (u64 * 64)(*[int(x) for x in sizes])
+        td = (u64 * 64)(*[int(x) for x in tids])
+        out = (u64 * 64)()
+        rc = lib.orc_group_malloc(self.h, lanes, sz, td, slot, out)
+        if rc != 0:
+            raise RuntimeError(f"orc_group_malloc failed: {rc}")
+        return list(out)
+
+    def group_free(self, lanes, offsets):
+        """free by the lanes of `lanes` together; offsets: 64 entries by lane."""
+        off = (u64 * 64)(*[int(x) for x in offsets])
+        return lib.orc_group_free(self.h, lanes, off)
 
     def snapshot(self):
         need = u64()
