@@ -234,6 +234,11 @@ class Gallatin:
         _capi.check("gal_group_replay", _lib.gal_group_replay(self._h, arr, n, out))
         return np.frombuffer(out, dtype=np.uint64, count=64 * n).copy().reshape(n, 64)
 
+    def group_replay_noalloc(self, d_ops, n, d_results):
+        """gal_group_replay_noalloc on device buffers the caller owns (device
+        addresses as ints); returns the status code (0, or GAL_ERR_*)."""
+        return _lib.gal_group_replay_noalloc(self._h, d_ops, n, d_results)
+
     def run_plain(self, n_threads, size, seed, verify=GAL_VERIFY_STORE):
         r = _capi.GalRunResult()
         _capi.check("gal_run_plain", _lib.gal_run_plain(self._h, n_threads, size, seed, verify, ctypes.byref(r)))

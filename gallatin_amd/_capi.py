@@ -17,6 +17,7 @@ GAL_ERR_OOM = 3
 GAL_ERR_HIP = 4
 GAL_ERR_BUFFER = 5
 GAL_ERR_STATE = 6
+GAL_ERR_TOO_LARGE = 7
 ERROR_NAMES = {
     GAL_OK: "ok",
     GAL_ERR_INVALID: "invalid argument",
@@ -24,7 +25,8 @@ ERROR_NAMES = {
     GAL_ERR_OOM: "out of device memory",
     GAL_ERR_HIP: "HIP runtime error",
     GAL_ERR_BUFFER: "buffer too small",
-    GAL_ERR_STATE: "allocator state error",
+    GAL_ERR_STATE: "allocator state error (or the handle is still in use)",
+    GAL_ERR_TOO_LARGE: "request above the entry point's fixed limit",
 }
 
 GAL_OP_MALLOC = 0
@@ -35,6 +37,7 @@ GAL_VERIFY_ATOMIC = 1
 GAL_FORM_PER_THREAD = 0
 GAL_FORM_BATCHED = 1
 GAL_FORM_GRAPH = 2
+GAL_NOALLOC_MAX_OPS = 8192
 
 u64 = ctypes.c_uint64
 u32 = ctypes.c_uint32
@@ -173,6 +176,7 @@ _SIGNATURES = {
     "gal_snapshot": (ctypes.c_int, [vp, vp, u64, ctypes.POINTER(u64)]),
     "gal_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64)]),
     "gal_group_replay": (ctypes.c_int, [vp, ctypes.POINTER(GalGroupOp), u64, ctypes.POINTER(u64)]),
+    "gal_group_replay_noalloc": (ctypes.c_int, [vp, vp, u64, vp]),
     "gal_errors": (ctypes.c_int, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
     "gal_error_sites": (ctypes.c_int, [vp, ctypes.POINTER(u32), u32]),
     "gal_stats": (ctypes.c_int, [vp, ctypes.POINTER(u32), u32]),

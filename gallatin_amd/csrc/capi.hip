@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <gallatin/allocators/global_allocator.hpp>
@@ -69,6 +70,8 @@ struct gal_allocator {
   // batches, serving loops and slab tables made on this handle and not yet
   // destroyed: they hold its stream and allocator, so the handle outlives them
   int children = 0;
+  // device address of the no-allocation scratch array on this handle's GPU
+  uint64_t* scratch = nullptr;
 };
 
 // a batch of requests and the device buffers that bin and serve it
@@ -127,7 +130,7 @@ struct impl_ops {
   int (*snapshot)(gal_allocator* h, uint8_t* out);
   int (*replay)(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t* d_results, uint32_t* d_status);
   int (*group_replay)(gal_allocator* h, const gal_group_op* d_ops, uint64_t n, uint64_t* d_results,
-                      uint32_t* d_status);
+                      uint32_t* d_status, uint64_t* d_freed);
   uint32_t (*path_of)(uint64_t size);  // 0 slice (incl. big slices), 1 whole block, 2 segments
   void (*launch_plain)(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify);
   void (*launch_mixed_malloc)(gal_allocator* h, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
@@ -211,39 +214,61 @@ __global__ void replay_kernel(A* a, const gal_op* ops, uint64_t n, uint64_t* res
 // One wavefront runs the group ops in order: the lanes of op i's mask reach
 // the malloc (or free) call together, with exactly that exec mask. Claims
 // use the reference's pinned-slot walk (no overflow-slot spreading), so the
-// results depend on the op stream only. results[i * 64 + l] was set to
-// GAL_NULL_OFFSET by the host for every entry; a free reads the result of an
-// earlier op of this wave, written by any of its lanes, after the
+// results depend on the op stream only. Every lane writes its own result
+// entry (GAL_NULL_OFFSET for lanes outside the op); a free reads the result
+// of an earlier op of this wave, written by any of its lanes, after the
 // workgroup-scope fence that ends each op.
+//
+// Before an op runs, its lanes check it: an unknown kind, an empty mask, or
+// a free of a result that failed, stop the replay (*status = op index + 1).
+// With `freed` (one u64 per op, zeroed: bit l set once result (op, l) has
+// been freed) the wave also checks what the host checks otherwise: a free
+// names a lane of an earlier malloc op, no result is freed twice, a free op
+// of several lanes frees no segment run, a malloc op's sizes are nonzero
+// and all on the slice path or all above it.
 template <class A>
 __global__ void __launch_bounds__(64) group_replay_kernel(A* a, const gal_group_op* ops, uint64_t n,
-                                                          uint64_t* results, uint32_t* status) {
+                                                          uint64_t* results, uint32_t* status, uint64_t* freed) {
   const uint32_t lane = wave::lane_id();
   char* base = a->table.memory;
   for (uint64_t i = 0; i < n; i++) {
     const gal_group_op* op = &ops[i];
     const uint64_t lanes = op->lanes;
     const uint32_t kind = op->kind;
-    bool bad = false;
-    if ((lanes >> lane) & 1ull) {
-      const uint64_t arg = op->arg[lane];
-      if (kind == GAL_OP_MALLOC) {
-        void* p = a->template malloc_at<false>(arg, op->tid[lane], op->slot);
-        results[i * 64 + lane] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
-      } else {
-        const uint64_t r = results[arg];
-        bad = r == GAL_NULL_OFFSET;
-        if (!bad) {
-          a->free(base + r);
-          results[i * 64 + lane] = 0;
-        }
+    const bool in = (lanes >> lane) & 1ull;
+    const uint64_t arg = in ? op->arg[lane] : 0;
+    bool bad = lanes == 0 || kind > GAL_OP_FREE;
+    uint32_t path = 0;
+    if (!bad && in && kind == GAL_OP_MALLOC) {
+      bad = freed && arg == 0;
+      path = A::request_path(arg);
+    } else if (!bad && in) {
+      if (freed) {
+        const uint64_t src = arg / 64, src_lane = arg % 64;
+        bad = arg >= i * 64 || ops[src].kind != GAL_OP_MALLOC || !((ops[src].lanes >> src_lane) & 1ull);
+        if (!bad && (lanes & (lanes - 1)) != 0) bad = A::request_path(ops[src].arg[src_lane]) == 2u;
+        if (!bad) bad = ((atom::fetch_or(&freed[src], 1ull << src_lane) >> src_lane) & 1ull) != 0;
       }
+      if (!bad) bad = results[arg] == GAL_NULL_OFFSET;
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (freed && kind == GAL_OP_MALLOC) {
+      const uint64_t large = wave::ballot(in && path != 0u), small = wave::ballot(in && path == 0u);
+      bad = bad || (large != 0 && small != 0);
+    }
     if (wave::ballot(bad)) {
-      if (lane == 0) *status = 1u;
+      if (lane == 0) atom::store(status, (uint32_t)(i + 1));
       return;
     }
+    if (!in) {
+      results[i * 64 + lane] = GAL_NULL_OFFSET;
+    } else if (kind == GAL_OP_MALLOC) {
+      void* p = a->template malloc_at<false>(arg, op->tid[lane], op->slot);
+      results[i * 64 + lane] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
+    } else {
+      a->free(base + results[arg]);
+      results[i * 64 + lane] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   }
 }
 
@@ -581,6 +606,47 @@ __global__ void global_poison_kernel(uint64_t size, int* codes) {
 
 inline unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
+// ---- scratch of the entry point that allocates nothing
+// (gal_group_replay_noalloc): 16 words of status (the first u32 is the
+// replay status), then one u64 per op whose bit l is set once result (op, l)
+// has been freed. Every GPU holds its own copy until the process ends; its
+// address on each device comes from hipGetSymbolAddress (the array's name is
+// only a host-side stand-in).
+constexpr uint64_t kScratchStatusWords = 16;
+static __device__ __align__(256) uint64_t scratch[kScratchStatusWords + GAL_NOALLOC_MAX_OPS];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint64_t* g_scratch[kMaxDevices] = {};
+
+// the scratch address on the current device, looked up once per device
+int scratch_on_current_device(uint64_t** out) {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return GAL_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GAL_ERR_HIP;
+    g_scratch[device] = (uint64_t*)p;
+  }
+  *out = g_scratch[device];
+  return GAL_OK;
+}
+
+__global__ void scratch_clear_kernel(uint64_t* p, uint64_t words) {
+  for (uint64_t i = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// true when p is device memory (or memory the device can address as its own)
+bool is_device_pointer(const void* p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
 // Threads per block of the one-request-per-thread kernels. A block's waves
 // hold their CU slots until the last of them ends, so larger blocks keep
 // fewer claimers in flight at once; on the 8 GiB mixed workload 1024-thread
@@ -691,18 +757,11 @@ struct impl {
   }
 
   static int group_replay(gal_allocator* h, const gal_group_op* d_ops, uint64_t n, uint64_t* d_results,
-                          uint32_t* d_status) {
-    group_replay_kernel<A><<<1, 64, 0, h->stream>>>((A*)h->dev, d_ops, n, d_results, d_status);
+                          uint32_t* d_status, uint64_t* d_freed) {
+    group_replay_kernel<A><<<1, 64, 0, h->stream>>>((A*)h->dev, d_ops, n, d_results, d_status, d_freed);
     return hipGetLastError() == hipSuccess ? GAL_OK : GAL_ERR_HIP;
   }
-  // the path malloc takes for a request (Gallatin::malloc_at's routing)
-  static uint32_t path_of(uint64_t size) {
-    const int bits = gu::get_first_bit_bigger(size < M ? M : size);
-    if (bits - A::kSmallestBits < (int)A::kNumTrees) return 0u;
-    const int block_tree = bits - A::kSmallestTreeBits;
-    if (block_tree < 0) return 0u;
-    return block_tree < (int)A::kNumTrees ? 1u : 2u;
-  }
+  static uint32_t path_of(uint64_t size) { return A::request_path(size); }
 
   static void launch_plain(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify) {
     plain_kernel<A><<<blocks_for(n, kClaimBlock), kClaimBlock, 0, h->stream>>>((A*)h->dev, n, size, seed, verify, h->d_counters);
@@ -1031,6 +1090,10 @@ static int make_handle(const impl_ops* ops, void* dev, gal_allocator** out) {
     h->tick_ns = 1e6 / (double)khz;
   int rc = ops->describe(h);
   if (rc != GAL_OK) return fail(rc);
+  // set-up of the no-allocation entry point: the scratch array's address on
+  // this handle's device (the current one)
+  rc = scratch_on_current_device(&h->scratch);
+  if (rc != GAL_OK) return fail(rc);
   *out = h;
   return GAL_OK;
 }
@@ -1198,7 +1261,7 @@ int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint
   if (rc == GAL_OK && (hipMemcpy(d_ops, ops, n * sizeof(gal_group_op), hipMemcpyHostToDevice) != hipSuccess ||
                        hipMemset(d_res, 0xFF, n * 64 * 8) != hipSuccess || hipMemset(d_status, 0, 4) != hipSuccess))
     rc = GAL_ERR_HIP;
-  if (rc == GAL_OK) rc = h->ops->group_replay(h, d_ops, n, d_res, d_status);
+  if (rc == GAL_OK) rc = h->ops->group_replay(h, d_ops, n, d_res, d_status, nullptr);
   if (rc == GAL_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = GAL_ERR_HIP;
   uint32_t status = 0;
   if (rc == GAL_OK && (hipMemcpy(results, d_res, n * 64 * 8, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -1209,6 +1272,24 @@ int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint
   if (d_res) (void)hipFree(d_res);
   if (d_status) (void)hipFree(d_status);
   return rc;
+}
+
+int gal_group_replay_noalloc(gal_allocator* h, const gal_group_op* d_ops, uint64_t n, uint64_t* d_results) {
+  if (!h || (n && (!d_ops || !d_results))) return GAL_ERR_INVALID;
+  if (n > GAL_NOALLOC_MAX_OPS) return GAL_ERR_TOO_LARGE;
+  if (!h->scratch) return GAL_ERR_STATE;
+  if (n == 0) return GAL_OK;
+  if (!is_device_pointer(d_ops) || !is_device_pointer(d_results)) return GAL_ERR_INVALID;
+  // the status words and this call's part of the freed-result bitmap start at zero
+  const uint64_t words = kScratchStatusWords + n;
+  scratch_clear_kernel<<<blocks_for(words, 256), 256, 0, h->stream>>>(h->scratch, words);
+  if (hipGetLastError() != hipSuccess) return GAL_ERR_HIP;
+  int rc = h->ops->group_replay(h, d_ops, n, d_results, (uint32_t*)h->scratch, h->scratch + kScratchStatusWords);
+  if (rc != GAL_OK) return rc;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  uint32_t status = 0;
+  if (hipMemcpy(&status, h->scratch, sizeof(status), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  return status == 0 ? GAL_OK : GAL_ERR_INVALID;
 }
 
 int gal_run_plain(gal_allocator* h, uint64_t n_threads, uint64_t size, uint64_t seed, uint32_t verify,

@@ -726,6 +726,16 @@ struct Gallatin {
     return offset_to_allocation(offset, tree);
   }
 
+  // The path malloc takes for a request: 0 slices (one, or 2^k top-class
+  // slices), 1 a whole block, 2 a run of segments (reference :1186-1276).
+  __host__ __device__ static uint32_t request_path(uint64_t size) {
+    const int bits = utils::get_first_bit_bigger(size < smallest ? smallest : size);
+    if (bits - kSmallestBits < (int)kNumTrees) return 0u;
+    const int block_tree = bits - kSmallestTreeBits;
+    if (block_tree < 0) return 0u;
+    return block_tree < (int)kNumTrees ? 1u : 2u;
+  }
+
   // Bytes between consecutive allocations of a run of `size`-byte requests
   // (0: the size takes the block or segment path, which has no runs).
   __host__ __device__ static uint64_t run_stride(uint64_t size) {

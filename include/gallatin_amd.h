@@ -41,7 +41,9 @@ extern "C" {
 #define GAL_ERR_OOM 3         /* device allocation failed or pool too small to boot */
 #define GAL_ERR_HIP 4         /* a HIP runtime call failed */
 #define GAL_ERR_BUFFER 5      /* caller buffer too small; *needed says how big */
-#define GAL_ERR_STATE 6       /* allocator detected an internal inconsistency */
+#define GAL_ERR_STATE 6       /* allocator detected an internal inconsistency, or the handle is
+                                 still used by a batch, serving loop or slab table */
+#define GAL_ERR_TOO_LARGE 7   /* request above a fixed limit of the entry point */
 
 typedef struct gal_allocator gal_allocator; /* opaque host handle */
 
@@ -174,6 +176,23 @@ typedef struct gal_group_op {
   uint64_t tid[64]; /* per lane: thread id fed to the hashed segment search */
 } gal_group_op;
 int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint64_t* results);
+
+/* gal_group_replay for callers that cannot allocate device memory: code
+ * inside a captured hipGraph, tight serving loops, harnesses that forbid
+ * hipMalloc. No counterpart in the reference. ops (n entries) and results
+ * (n * 64 entries, every one written up to the first rejected op) are device
+ * memory that the caller owns; host pointers are refused (GAL_ERR_INVALID).
+ * The rules of gal_group_replay are checked on the device, with a bitmap of
+ * freed results held in a fixed array of the library in device memory: one
+ * copy per GPU, kept until the process ends, room for GAL_NOALLOC_MAX_OPS
+ * ops (more: GAL_ERR_TOO_LARGE, nothing runs). Calls on one GPU share that
+ * array, so calls that overlap in time on one GPU race. The call clears the
+ * part it uses with a kernel, runs on the handle's stream and waits for it;
+ * it calls no hipMalloc, hipFree or other allocating routine. Returns
+ * GAL_ERR_INVALID if an op breaks the rules: the ops before it have run, it
+ * and the later ones have not. */
+#define GAL_NOALLOC_MAX_OPS 8192u
+int gal_group_replay_noalloc(gal_allocator* h, const gal_group_op* d_ops, uint64_t n, uint64_t* d_results);
 
 /* Device error counters: internal inconsistencies (where the reference would
  * trap) and rejected frees of pointers the allocator does not own. */
