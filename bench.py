@@ -11,6 +11,8 @@ the device from the seed, so nothing is transferred inside the timed region.
 Prints ONE JSON line. Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import sys
@@ -21,25 +23,47 @@ sys.path.insert(0, ROOT)
 
 METRIC = "device mallocs+frees/sec on 1 MI355X (mixed sizes); p99 alloc latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per malloc+free pair on the slice path: metadata 26 B
-# (8 B pinned-block pointer read, 4+4 B malloc-counter RMW, 4+4 B free-counter
-# RMW, 2 B chunk-id read) + payload 16 B (8 B tag write, 8 B tag read).
-# Malloc kernel share: 8 + 8 + 8 = 24 B; free kernel share: 8 + 2 + 8 = 18 B.
-BYTES_MALLOC_SIDE = 24
-BYTES_FREE_SIDE = 18
-# HBM bytes per launch of the workload kernels at the default workload, from
-# rocprofv3 FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch, averaged over a
-# step) scaled by what the counters read for a 1 GiB device copy (fetch x2.0,
-# write x1.0): profiles/mixed_slices_8GiB_hbm_traffic.md.
-MEASURED_TRAFFIC_BYTES = {"mixed_malloc_kernel": 196.0e6, "mixed_free_kernel": 351.6e6}
-# Agent-scope atomic requests per launch at the default workload, from a
-# rocprofv3 TCC_ATOMIC_sum run (averaged over a step). Every such atomic is
-# performed beyond the L2 (TCC_EA0_ATOMIC_sum reads the same). One request
-# serves the lanes of one instruction that fall in one 64-B span (16 lanes on
-# consecutive 4-B words); a single-lane atomic is one request:
-# profiles/mixed_slices_8GiB_tcc_atomics.md.
-MEASURED_ATOMIC_REQUESTS = {"mixed_malloc_kernel": 324070, "mixed_free_kernel": 205989}
+# Algorithmic bytes per malloc+free pair on the slice path, one request per
+# lane (no group sharing), from the metadata layout
+# (gallatin_amd/include/gallatin/allocators/memory_table.hpp, gallatin.hpp):
+SLOT_ENTRY = 4       # u32 pinned-slot entry, read by the claim
+MALLOC_COUNTER = 8   # u64 Block::malloc_counter, read and written by the claim's atomic add
+FREE_COUNTER = 4     # u32 Block::free_counter, read and written by the free's atomic add
+CHUNK_ID = 4         # u32 chunk id of the segment, read by free
+TAG = 8              # u64 payload tag, written after malloc and read before free
+BYTES_MALLOC_SIDE = SLOT_ENTRY + 2 * MALLOC_COUNTER + TAG  # 28 B per successful malloc
+BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
+# Counter figures of the default workload's kernels, per launch averaged over
+# one step, measured by rocprofv3 on the kernel sources whose digest is
+# `sources_sha` (the digest of KERNEL_SOURCES below; the figures are reported
+# only while the sources still have that digest):
+#   traffic: FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch) scaled by what
+#     the counters read for a 1 GiB device copy (fetch x2.0, write x1.0),
+#     profiles/mixed_slices_8GiB_hbm_traffic.md;
+#   atomic_requests: TCC_ATOMIC_sum (every agent-scope atomic is performed
+#     beyond the L2: TCC_EA0_ATOMIC_sum reads the same; one request serves
+#     the lanes of one instruction within one 64-B span),
+#     profiles/mixed_slices_8GiB_tcc_atomics.md.
+MEASURED = {
+    "sources_sha": "",
+    "commit": "",
+    "traffic": {"mixed_malloc_kernel": None, "mixed_free_kernel": None},
+    "atomic_requests": {"mixed_malloc_kernel": None, "mixed_free_kernel": None},
+}
+KERNEL_SOURCES = ("gallatin_amd/csrc/*.hip", "gallatin_amd/csrc/*.inl", "gallatin_amd/include/gallatin/*.hpp",
+                  "gallatin_amd/include/gallatin/allocators/*.hpp")
 DEFAULT_WORKLOAD = (8, 22, 10)  # pool GiB, log2 threads per round, rounds
+
+
+def sources_sha():
+    """Digest of the kernel sources the library is built from."""
+    h = hashlib.sha256()
+    for pattern in KERNEL_SOURCES:
+        for path in sorted(glob.glob(os.path.join(ROOT, pattern))):
+            h.update(os.path.relpath(path, ROOT).encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def log(msg):
@@ -71,6 +95,51 @@ def cpu_baseline(pool, seed, threads, rounds):
 
 def rate(ops, ms):
     return ops / (ms * 1e-3) if ms > 0 else None
+
+
+def plain(seed, launches, cpu):
+    """The reference's global test kernel at full size: a 1 GiB global
+    allocator, one kernel of 2^20 threads, each global_malloc(16 B), writes a
+    tag drawn from (seed, thread), checks it, global_free."""
+    import gallatin_amd as ga
+
+    n = 1 << 20
+    ga.init_global_allocator(1 << 30, seed, print_info=False)
+    try:
+        ga.global_plain(n, 16, seed, ga.GAL_VERIFY_STORE)  # warm-up
+        rs = [ga.global_plain(n, 16, seed + 1 + k, ga.GAL_VERIFY_STORE) for k in range(launches)]
+        dev, _ = ga.global_handles()
+        errors = dev.errors()
+    finally:
+        ga.free_global_allocator()
+    ops = sum(r["mallocs"] + r["frees"] for r in rs)
+    ms = sum(r["total_ms"] for r in rs)
+    out = {
+        "workload": "1 GiB global allocator (Gallatin<16MiB,16,4096>), one kernel of 2^20 threads: "
+        "global_malloc(16 B), tag write, check, global_free",
+        "launches": launches,
+        "mallocs_frees_per_s": rate(ops, ms),
+        "kernel_avg_us": ms / launches * 1e3,
+        "misses": sum(r["misses"] for r in rs),
+        "tag_errors": sum(r["tag_errors"] for r in rs),
+        "alloc_errors": sum(r["alloc_errors"] for r in rs),
+        "errors": errors,
+    }
+    if cpu:
+        from oracle import oracle
+
+        sysm = oracle.bench_sysmalloc(1 << 22, 0, seed, size=16)
+        out["cpu_baseline"] = {
+            "value": sysm["ops"] / sysm["seconds"] if sysm["seconds"] > 0 else None,
+            "unit": "mallocs+frees/s",
+            "cores": sysm["threads"],
+            "kind": "system",
+            "sample": f"glibc malloc(16)/tag write/check/free, {1 << 22} per thread on every host thread",
+            "p50_ns": sysm["p50_ns"],
+            "p99_ns": sysm["p99_ns"],
+        }
+    log(f"[bench] plain: {out}")
+    return out
 
 
 def forms(seed):
@@ -128,13 +197,21 @@ def forms(seed):
     base = out["per_thread"]["mallocs_frees_per_s"]
     out["vs_per_thread"] = {k: (out[k]["mallocs_frees_per_s"] / base if out[k]["mallocs_frees_per_s"] else None)
                             for k in ("batched", "graph", "serving", "serving_short")}
+    # the serving forms over their whole run (ramp-down at the end included)
+    out["vs_per_thread_whole_run"] = {k: out[k]["whole_run_mallocs_frees_per_s"] / base
+                                      for k in ("serving", "serving_short")}
     return out
 
 
-def serving(seed, requests, rate_per_s):
+def serving(seed, requests, rate_per_s, run=64, sample=20000, grace_s=60.0):
     """Sequences arriving by a Poisson process, growing by 16 KiB pages, admitted
-    by free pool bytes, over a 200 GiB pool."""
+    by free pool bytes, over a 200 GiB pool. `run` pages are claimed per
+    allocator call (1: one malloc per page). The checksums of a seeded sample
+    of requests are checked against a host computation."""
+    import numpy as np
+
     import gallatin_amd as ga
+    from tests import serving_ref
 
     g = ga.Gallatin.generate_on_device(200 << 30, seed)
     try:
@@ -142,11 +219,19 @@ def serving(seed, requests, rate_per_s):
                                     page_bytes=16384, seed=seed)
         with ga.Serving(g, len(reqs)) as s:
             s.load(reqs)
-            r = s.run(slots=16384, run=64, deadline_s=float(reqs["arrival_ns"][-1]) * 1e-9 + 60.0)
+            r = s.run(slots=16384, run=run, deadline_s=float(reqs["arrival_ns"][-1]) * 1e-9 + grace_s)
+            sums, _, _ = s.results()
+        pick = np.random.default_rng(seed).choice(len(reqs), size=min(sample, len(reqs)), replace=False)
+        done = pick[sums[pick] != 0]
+        expect = serving_ref.checksums_c(reqs["id"][done], reqs["pages"][done], reqs["page_bytes"][done])
         sec = r["kernel_ms"] * 1e-3
         out = {
             "workload": f"200 GiB pool, {requests} requests, Poisson {rate_per_s:.0f}/s, lognormal length (median "
-            "1024, sigma 1, cap 32768) of 16 KiB pages, admission by free pool bytes, 16384 slots, runs of 64",
+            f"1024, sigma 1, cap 32768) of 16 KiB pages, admission by free pool bytes, 16384 slots, runs of {run}",
+            "run": run,
+            "checksum_sample": int(len(pick)),
+            "checksum_checked": int(len(done)),
+            "checksum_mismatches": int(np.count_nonzero(sums[done] != expect)),
             "arrival_span_s": float(reqs["arrival_ns"][-1]) * 1e-9,
             "requests_per_s": r["requests_done"] / sec if sec > 0 else None,
             "page_mallocs_frees_per_s": rate(r["page_mallocs"] + r["page_frees"], r["kernel_ms"]),
@@ -341,7 +426,9 @@ def main():
         dom, n_launch, dom_ms, dom_bytes = "mixed_free_kernel", (rounds + 1) * args.steps, free_ms, frees * BYTES_FREE_SIDE
     achieved = (dom_bytes / n_launch) / ((dom_ms / n_launch) * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     default = (args.pool_gib, args.threads_log2, args.rounds) == DEFAULT_WORKLOAD
-    traffic = MEASURED_TRAFFIC_BYTES[dom] if default else None
+    sha = sources_sha()
+    current = default and sha == MEASURED["sources_sha"]
+    traffic = MEASURED["traffic"][dom] if current else None
 
     import numpy as np
 
@@ -390,6 +477,8 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": dom_bytes / n_launch,
             "kernel_avg_us": dom_ms / n_launch * 1e3,
+            "counters": {"sources_sha": sha, "measured_on_sources": MEASURED["sources_sha"] or None,
+                         "commit": MEASURED["commit"] or None, "current": current},
         },
     }
     g.free_on_device()
@@ -402,12 +491,12 @@ def main():
         "wave_hot_65536_words": ga.atomic_rate(ga.ATOMIC_WAVE_HOT, 1 << 22, 16, 65536),
     }
     log(f"[bench] atomic rates {out['atomic_rates_per_s']}")
-    if default:
+    if current and MEASURED["atomic_requests"][out["roofline"]["kernel"]]:
         # the same kernel against the atomic request rate this GPU sustains on
         # distinct words (the probe's lane rate / 16 lanes per request) and,
         # for one hot line, the single-line rate
         dom = out["roofline"]["kernel"]
-        reqs = MEASURED_ATOMIC_REQUESTS[dom]
+        reqs = MEASURED["atomic_requests"][dom]
         achieved_req = reqs / (out["roofline"]["kernel_avg_us"] * 1e-6)
         peak_req = out["atomic_rates_per_s"]["distinct_lanes"] / 16
         out["atomic_roofline"] = {
@@ -422,8 +511,11 @@ def main():
         }
 
     if not args.headline_only:
+        out["plain"] = plain(args.seed, 5, not args.no_cpu_baseline)
         out["forms"] = forms(args.seed)
         out["serving"] = serving(args.seed, args.serving_requests, args.serving_rate)
+        # one malloc per page: the same requests, the loop bounded at 5 s past the last arrival
+        out["serving_per_page"] = serving(args.seed, args.serving_requests, args.serving_rate, run=1, grace_s=5.0)
         out["blocks"] = blocks(args.seed)
         out["slab"] = slab(args.seed)
 

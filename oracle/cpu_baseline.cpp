@@ -4,6 +4,7 @@
  * TEST INFRASTRUCTURE, NOT PRODUCT CODE.
  *  orc_bench_mixed     the oracle replaying the mixed-size workload on one core
  *  orc_bench_sysmalloc glibc malloc/free on all host cores, same size law
+ *  orc_bench_sysmalloc_size the same at one fixed size (the plain workload's 16 B)
  *
  * The mixed-size request stream is defined by mixed_key() below; the GPU
  * workload kernels generate the identical stream (request size of thread i in
@@ -83,6 +84,13 @@ extern "C" int orc_bench_mixed(uint64_t pool_bytes, uint64_t seed, uint64_t thre
 
 extern "C" int orc_bench_sysmalloc(uint64_t ops_per_thread, uint32_t threads, uint64_t seed, uint64_t* ops_done,
                                    double* seconds, double* p50_ns, double* p99_ns, uint32_t* threads_used) {
+  return orc_bench_sysmalloc_size(ops_per_thread, threads, seed, 0, ops_done, seconds, p50_ns, p99_ns,
+                                  threads_used);
+}
+
+extern "C" int orc_bench_sysmalloc_size(uint64_t ops_per_thread, uint32_t threads, uint64_t seed, uint64_t fixed_size,
+                                        uint64_t* ops_done, double* seconds, double* p50_ns, double* p99_ns,
+                                        uint32_t* threads_used) {
   if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
   std::vector<std::vector<float>> lat(threads);
   std::vector<uint64_t> bad(threads, 0);
@@ -96,7 +104,7 @@ extern "C" int orc_bench_sysmalloc(uint64_t ops_per_thread, uint32_t threads, ui
     }
     for (uint64_t i = 0; i < ops_per_thread; i++) {
       state = sm64(state);
-      size_t size = (size_t)16 << (state % 9);
+      size_t size = fixed_size ? (size_t)fixed_size : (size_t)16 << (state % 9);
       uint64_t tag = state | 1ull;
       bool sample = (i & 63) == 0;
       std::chrono::steady_clock::time_point a;

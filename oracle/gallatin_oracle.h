@@ -85,6 +85,10 @@ int orc_bench_mixed(uint64_t pool_bytes, uint64_t seed, uint64_t threads_per_rou
 int orc_bench_sysmalloc(uint64_t ops_per_thread, uint32_t threads, uint64_t seed,
                         uint64_t* ops_done, double* seconds, double* p50_ns,
                         double* p99_ns, uint32_t* threads_used);
+/* the same with every request fixed_size bytes (0: the log-uniform law) */
+int orc_bench_sysmalloc_size(uint64_t ops_per_thread, uint32_t threads, uint64_t seed, uint64_t fixed_size,
+                             uint64_t* ops_done, double* seconds, double* p50_ns, double* p99_ns,
+                             uint32_t* threads_used);
 
 #ifdef __cplusplus
 }

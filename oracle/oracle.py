@@ -61,6 +61,11 @@ _SIG = {
     "orc_snapshot": (ctypes.c_int, [vp, vp, u64, pu64]),
     "orc_faults": (u64, [vp]),
     "orc_bench_mixed": (ctypes.c_int, [u64, u64, u64, u32, pu64, pu64, ctypes.POINTER(ctypes.c_double)]),
+    "orc_bench_sysmalloc_size": (
+        ctypes.c_int,
+        [u64, u32, u64, u64, pu64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)],
+    ),
     "orc_bench_sysmalloc": (
         ctypes.c_int,
         [
@@ -238,9 +243,11 @@ def bench_mixed(pool_bytes, seed, threads_per_round, rounds):
     return ops.value, miss.value, sec.value
 
 
-def bench_sysmalloc(ops_per_thread, threads=0, seed=1):
+def bench_sysmalloc(ops_per_thread, threads=0, seed=1, size=0):
+    """glibc malloc/free on `threads` host threads (0: all); size 0 draws the
+    log-uniform 16 B-4 KiB law, otherwise every request is `size` bytes."""
     ops, sec, p50, p99, used = u64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), u32()
-    rc = lib.orc_bench_sysmalloc(ops_per_thread, threads, seed, ctypes.byref(ops), ctypes.byref(sec),
-                                 ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(used))
+    rc = lib.orc_bench_sysmalloc_size(ops_per_thread, threads, seed, size, ctypes.byref(ops), ctypes.byref(sec),
+                                      ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(used))
     return {"ops": ops.value, "seconds": sec.value, "p50_ns": p50.value, "p99_ns": p99.value,
             "threads": used.value, "rc": rc}

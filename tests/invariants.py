@@ -37,13 +37,9 @@ def check_live(offsets, sizes, pool_bytes, seg, mn, mx, route):
     sizes = np.asarray(sizes, dtype=np.uint64)
     if len(offsets) == 0:
         return
-    cache = {}
-    granted = np.zeros(len(sizes), dtype=np.uint64)
-    align = np.zeros(len(sizes), dtype=np.uint64)
-    for i, s in enumerate(sizes.tolist()):
-        if s not in cache:
-            cache[s] = granted_bytes(seg, mn, mx, s, route)
-        granted[i], align[i] = cache[s]
+    uniq, inv = np.unique(sizes, return_inverse=True)
+    table = np.array([granted_bytes(seg, mn, mx, int(s), route) for s in uniq.tolist()], dtype=np.uint64)
+    granted, align = table[inv, 0], table[inv, 1]
     assert np.all(offsets % align == 0), "misaligned allocation"
     assert np.all(offsets + granted <= np.uint64(pool_bytes)), "allocation outside the pool"
     order = np.argsort(offsets, kind="stable")

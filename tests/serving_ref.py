@@ -73,3 +73,19 @@ def expected_checksum_scalar(rid, pages, page_bytes):
         for k in range(1, w):
             h = splitmix64_int(h ^ word(s, k))
     return splitmix64_int(h ^ pages)
+
+
+def checksums_c(ids, pages, page_bytes):
+    """The same computed by tests/native/serving_check.c (large samples)."""
+    import ctypes
+    import os
+
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "build",
+                                   "libserving_check.so"))
+    ids = np.ascontiguousarray(ids, dtype=np.uint64)
+    pages = np.ascontiguousarray(pages, dtype=np.uint32)
+    page_bytes = np.ascontiguousarray(page_bytes, dtype=np.uint32)
+    out = np.zeros(len(ids), dtype=np.uint64)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    lib.serving_checksums(p(ids), p(pages), p(page_bytes), ctypes.c_uint64(len(ids)), p(out))
+    return out

@@ -113,3 +113,14 @@ def test_checksum_depends_on_every_read():
     assert base != serving_ref.expected_checksum_scalar(12, 6, 64)
     assert base != serving_ref.expected_checksum_scalar(11, 6, 32)
     assert serving_ref.splitmix64_int(0) == 0xE220A8397B1DCDAF  # splitmix64 known answer (Vigna)
+
+
+def test_checksum_c_matches_numpy():
+    rng = np.random.default_rng(3)
+    n = 300
+    ids = rng.integers(0, 2**63, n, dtype=np.uint64)
+    pages = rng.integers(1, 200, n)
+    page_bytes = rng.choice([16, 24, 32, 48, 64, 4096, 16384], n)
+    expect = serving_ref.expected_checksums(ids, pages, page_bytes)
+    got = serving_ref.checksums_c(ids, pages, page_bytes)
+    assert np.array_equal(got, expect)
