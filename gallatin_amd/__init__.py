@@ -160,6 +160,7 @@ class Gallatin:
             "blocks_available": list(r.blocks_available[:t]),
             "blocks_in_use": list(r.blocks_in_use[:t]),
             "overhead_bytes": r.overhead_bytes,
+            "empty_pinned_slots": [int(ctypes.c_int64(x).value) for x in r.empty_pinned_slots[:t]],
         }
 
     @property

@@ -161,6 +161,7 @@ class GalFillReport(ctypes.Structure):
         ("blocks_available", u64 * 32),
         ("blocks_in_use", u64 * 32),
         ("overhead_bytes", u64),
+        ("empty_pinned_slots", u64 * 32),
     ]
 
 

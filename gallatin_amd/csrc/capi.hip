@@ -812,6 +812,10 @@ struct impl {
       out->blocks_in_use[c] += (uint64_t)(nb - segs[x].active);
     }
     out->overhead_bytes = h->info.metadata_bytes;
+    std::vector<int32_t> words(A::kNumTrees * ga::kCtlStride);
+    if (hipMemcpy(words.data(), host.class_words, words.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return GAL_ERR_HIP;
+    for (uint32_t c = 0; c < A::kNumTrees; c++) out->empty_pinned_slots[c] = (uint64_t)(int64_t)words[c * ga::kCtlStride];
     return GAL_OK;
   }
   static void launch_churn(gal_allocator* h, const gal_op* d_ops, uint64_t n, uint64_t seed, uint64_t* d_results) {

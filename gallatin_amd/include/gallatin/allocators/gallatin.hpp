@@ -122,6 +122,11 @@ struct Gallatin {
   uint32_t slot_offset[kNumTrees];
   uint32_t usable_slots[kNumTrees];  // usable_slots_of each class for this pool
   uint32_t* control;              // control_word array
+  // per class, one 128-B line; word 0: how many of the class's pinned slots
+  // hold 0 (a replacement or refill found no memory). Changed only on those
+  // rare paths, lowered before and raised after the slot itself changes, so
+  // it never exceeds the true count (read as signed: it can dip below 0).
+  int32_t* class_words;
   uint64_t seed;
   // device allocations, for free_on_device
   void* owned[6];
@@ -144,6 +149,7 @@ struct Gallatin {
   __device__ void count_error(uint32_t site) { count_error_at(control, site); }
 
   __device__ uint32_t* slot_ptr(uint32_t tree, uint32_t index) const { return &slots[slot_offset[tree] + index]; }
+  __device__ int32_t* empty_pinned_word(uint32_t tree) const { return class_words + tree * kCtlStride; }
 
   // ------------------------------------------------- segments and blocks
   // attach the lowest free segment to class `tree` (reference :1465-1510)
@@ -311,6 +317,7 @@ struct Gallatin {
     const int64_t fresh = request_new_block_from_tree(tree, tid, last);
     atom::fence_release();  // the block's counter reset precedes its publication
     atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
+    if (fresh < 0 && index < pinned_slots_of(tree)) atom::add(empty_pinned_word(tree), 1);
     count_stat(control, kStatReplaceTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
     if (last) gather_after_last(tree);
     return fresh >= 0;
@@ -320,13 +327,19 @@ struct Gallatin {
   // memory is still short
   __device__ __noinline__ bool refill_slot(uint32_t tree, uint32_t index, uint64_t tid) {
     uint32_t* s = slot_ptr(tree, index);
-    if (atom::cas(s, 0u, kSlotBusy) != 0u) return true;  // another group is on it
+    const bool pinned = index < pinned_slots_of(tree);
+    if (pinned) atom::sub(empty_pinned_word(tree), 1);
+    if (atom::cas(s, 0u, kSlotBusy) != 0u) {  // another group is on it
+      if (pinned) atom::add(empty_pinned_word(tree), 1);
+      return true;
+    }
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatRefill);
     bool last = false;
     const int64_t fresh = request_new_block_from_tree(tree, tid, last);
     atom::fence_release();
     atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
+    if (fresh < 0 && pinned) atom::add(empty_pinned_word(tree), 1);
     count_stat(control, kStatRefillTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
     if (last) gather_after_last(tree);
     return fresh >= 0;
@@ -424,28 +437,45 @@ struct Gallatin {
     // always for a single lane) the reference's walk over the pinned slots.
     bool placed = false;
     bool dry = false;  // the leader found the class out of memory in this pass
-    if (spread && attempt == 0 && usable > pinned_slots_of(tree) && (looking & (looking - 1)) != 0) {
-      const uint32_t si = lead_hint % usable;
-      // read with the slot, in the same round trip: the class's tree summary
-      // and the free-segment gauge
+    //
+    // A class whose usable slots are its pinned slots (the two smallest
+    // classes) takes the same first look at slot hint % pinned, where the
+    // walk would start: a block there is the walk's own result, and when the
+    // class is out of memory with every pinned slot holding 0 the group fails
+    // at once (the walk would end on an empty slot whose refill fails).
+    const uint32_t pinned = pinned_slots_of(tree);
+    if (spread && attempt == 0 && (looking & (looking - 1)) != 0) {
+      const bool spreads = usable > pinned;
+      const uint32_t si = lead_hint % (spreads ? usable : pinned);
+      // read with the slot, in the same round trip: the class's tree summary,
+      // the free-segment gauge and the count of empty pinned slots
       const uint64_t class_top = sub_trees[tree].top_word();
       const uint32_t free_segments = atom::load(ctl_word(control, kCtlFreeSegs));
+      const int32_t empty_pinned = atom::load(empty_pinned_word(tree));
       const uint32_t se = atom::load(slot_ptr(tree, si));
+      // The class's tree empty (its summaries are exact: it changes only by
+      // insert_force_update/remove) and no free segment left is out of
+      // memory without the search: the search would find nothing either.
+      const bool tree_dry = sub_trees[tree].top_looks_empty(class_top) && free_segments == 0u;
+      // Out of memory, and every pinned slot holds 0 as well: a walk over the
+      // pinned slots would find no block.
+      const bool all_empty = empty_pinned >= (int32_t)pinned;
       if (se != 0u && se != kSlotBusy) {
         index = si;
         entry = se;
         placed = true;
-      } else if (se == 0u) {
-        // The class's tree empty (its summaries are exact: it changes only by
-        // insert_force_update/remove) and no free segment left is out of
-        // memory without the search: the search would find nothing either.
-        dry = (sub_trees[tree].top_looks_empty(class_top) && free_segments == 0u) ||
-              class_out_of_memory(tree, tid);
-        if (!dry) {
+      } else if (se == 0u && spreads) {
+        dry = tree_dry || class_out_of_memory(tree, tid);
+        if (!dry || all_empty) {
           index = si;
           entry = 0u;
           placed = true;
         }
+      } else if (se == 0u && tree_dry && all_empty) {
+        dry = true;
+        index = si;
+        entry = 0u;
+        placed = true;
       }
     }
     placed = wave::shfl((uint32_t)placed, leader) != 0u;
@@ -1018,7 +1048,8 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint
   const size_t off_segs = off_queues + ((nblocks * 4 + 255) / 256) * 256;
   const size_t off_slots = off_segs + nseg * sizeof(segment_counters);
   const size_t off_ctl = off_slots + ((total_slots() * 4 + 255) / 256) * 256;
-  const size_t table_bytes = off_ctl + kCtlWords * kCtlStride * 4;
+  const size_t off_class = off_ctl + kCtlWords * kCtlStride * 4;
+  const size_t table_bytes = off_class + kNumTrees * kCtlStride * 4;
 
   auto fail = [&]() -> my_type* {
     if (veb) (void)hipFree(veb);
@@ -1068,6 +1099,7 @@ __host__ Gallatin<S, M, B>* Gallatin<S, M, B>::generate(uint64_t max_bytes, uint
   host.table.num_segments = nseg;
   host.slots = (uint32_t*)(tm + off_slots);
   host.control = (uint32_t*)(tm + off_ctl);
+  host.class_words = (int32_t*)(tm + off_class);
   uint32_t off = 0;
   for (uint32_t c = 0; c < kNumTrees; c++) {
     host.slot_offset[c] = off;

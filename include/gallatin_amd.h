@@ -138,6 +138,10 @@ typedef struct gal_fill_report {
   uint64_t blocks_available[32];
   uint64_t blocks_in_use[32];
   uint64_t overhead_bytes; /* device memory held besides the pool */
+  /* per class: pinned slots the allocator counts as holding no block (its
+   * fast out-of-memory test; equal to the slots that hold 0 whenever no call
+   * is in flight) */
+  uint64_t empty_pinned_slots[32];
 } gal_fill_report;
 int gal_fill(gal_allocator* h, gal_fill_report* out);
 int gal_snapshot(gal_allocator* h, void* buf, uint64_t cap, uint64_t* needed);

@@ -58,6 +58,24 @@ def check_live(offsets, sizes, pool_bytes, seg, mn, mx, route):
         )
 
 
+def pinned_slots_of(c):
+    """pinned slots of class c: 128 >> c, at least 4 (reference shared_block_storage.cuh:251-274)"""
+    return max(4, 128 >> c)
+
+
+def empty_pinned_counts(image):
+    """Per class, the pinned slots holding 0 in a snapshot (the slot region of
+    a class is its pinned slots followed by overflow slots up to 64)."""
+    s = snap.parse(image)
+    slots = s["slots"].astype(np.int64)
+    out, base = [], 0
+    for c in range(s["num_trees"]):
+        n = pinned_slots_of(c)
+        out.append(int(np.count_nonzero(slots[base:base + n] == 0)))
+        base += max(n, 64)
+    return out
+
+
 def bits_of(words):
     out = []
     for w, val in enumerate(np.asarray(words, dtype=np.uint64).tolist()):
