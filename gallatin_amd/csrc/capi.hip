@@ -691,7 +691,8 @@ struct impl {
     uint64_t bytes = words * 8 + ((nseg * 2 + 7) / 8) * 8 + nblocks * 12 + nseg * 16 + i.total_slots * 4;
     i.snapshot_bytes = ((bytes + 7) / 8) * 8;
     i.metadata_bytes =
-        words * 8 + nseg * 4 + nblocks * (sizeof(ga::Block) + 4) + nseg * 16 + i.total_slots * 4 + sizeof(A);
+        words * 8 + nseg * 4 + nblocks * (sizeof(ga::Block) + 4) + nseg * 16 + i.total_slots * ga::kSlotStride * 4 +
+        (ga::kCtlWords + A::kNumTrees) * ga::kCtlStride * 4 + sizeof(A);
     h->control = host.control;
     return GAL_OK;
   }
@@ -744,7 +745,13 @@ struct impl {
     p += nblocks * 4;
     if (hipMemcpy(p, host.table.segs, nseg * 16, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
     p += nseg * 16;
-    if (hipMemcpy(p, host.slots, A::total_slots() * 4, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+    {
+      // one entry per 128-B line on the device, packed in the image
+      std::vector<uint32_t> lines((size_t)A::total_slots() * ga::kSlotStride);
+      if (hipMemcpy(lines.data(), host.slots, lines.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return GAL_ERR_HIP;
+      for (uint32_t k = 0; k < A::total_slots(); k++) std::memcpy(p + 4 * k, &lines[(size_t)k * ga::kSlotStride], 4);
+    }
     p += A::total_slots() * 4;
     uint64_t used = (uint64_t)(p - out);
     if (h->info.snapshot_bytes > used) std::memset(p, 0, h->info.snapshot_bytes - used);
