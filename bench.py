@@ -373,9 +373,10 @@ def main():
     g = ga.Gallatin.generate_on_device(pool, args.seed)
 
     def step(seed, tag, rare_paths=False):
-        """One step: reset, every round (malloc kernel + free kernel), drain.
-        rare_paths: also read the allocator's rare-path counters after each
-        round for the log (one more device read per round; warm-up only)."""
+        """One step, round by round for the log: reset, every round (malloc
+        kernel + free kernel, then a device read of the round's counters),
+        drain. rare_paths: also read the allocator's rare-path counters after
+        each round (one more device read per round). Used for the warm-up."""
         g.mixed_reset(threads, rounds, seed)
         total = None
         before = g.stats() if rare_paths else None
@@ -400,7 +401,10 @@ def main():
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        results.append(step(args.seed + k, f"step {k}"))
+        # the same launches in one call: every round's malloc and free kernel
+        # and the drain back to back on the library's stream, one
+        # synchronisation at the end
+        results.append(g.run_mixed(threads, rounds, args.seed + k))
     t1 = time.perf_counter()
     for k, r in enumerate(results):
         log(f"[bench] step {k}: {r}")

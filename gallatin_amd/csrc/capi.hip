@@ -55,9 +55,16 @@ struct gal_allocator {
   hipEvent_t ev[2 * kEventPairs] = {};
   run_counters* d_counters = nullptr;
   uint32_t* control = nullptr;  // device control words of the allocator
-  // mixed workload state
+  // mixed workload state: the round's malloc results (one pointer per
+  // thread), and the survivors of the rounds so far in per-wave regions of
+  // packed entries (double buffered, a count per region; mixed_free_kernel)
   uint64_t* d_live = nullptr;
   uint64_t live_cap = 0;
+  uint64_t* d_regions[2] = {nullptr, nullptr};
+  uint32_t* d_region_counts[2] = {nullptr, nullptr};
+  uint64_t region_cap = 0;  // regions per buffer
+  uint64_t n_regions = 0;   // regions of buffer `cur` that hold the live set
+  int cur = 0;
   uint64_t threads_per_round = 0;
   uint32_t rounds = 0;
   uint32_t rounds_done = 0;
@@ -135,8 +142,8 @@ struct impl_ops {
   void (*launch_plain)(gal_allocator* h, uint64_t n, uint64_t size, uint64_t seed, uint32_t verify);
   void (*launch_mixed_malloc)(gal_allocator* h, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
                               uint32_t verify, uint32_t* lat);
-  void (*launch_mixed_free)(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
-                            bool all);
+  void (*launch_mixed_free)(gal_allocator* h, uint64_t n_fresh, uint64_t fresh_base, uint32_t round, uint64_t seed,
+                            uint32_t verify, bool all);
   void (*print)(gal_allocator* h);
   int (*fill)(gal_allocator* h, gal_fill_report* out);
   // batches (batch.inl)
@@ -348,7 +355,7 @@ __global__ void mixed_malloc_kernel(A* a, uint64_t* live, uint64_t base, uint64_
     const uint64_t t1 = wall_ticks();
     // one latency sample per wave: the wave's time inside malloc
     if (lat && (tid & 63) == 0) lat[(base + tid) >> 6] = (uint32_t)(t1 - t0);
-    live[base + tid] = (uint64_t)p;
+    live[tid] = (uint64_t)p;
     if (p) {
       got = true;
       bad = !tag_in(p, gw::payload_tag(seed, base + tid), verify);
@@ -359,15 +366,24 @@ __global__ void mixed_malloc_kernel(A* a, uint64_t* live, uint64_t base, uint64_
   count(c, got, false, miss, bad);
 }
 
-// Each wave scans 64 * kFreeScanIters consecutive entries of the live array
-// and packs the pointers it frees (live, and picked by the round's coin)
-// into full 64-lane batches in LDS before calling free, so that every free
-// call runs with a full wave (one release fence per 64 frees) however sparse
-// the live array has become.
+// The live table of the mixed workload. The malloc kernel of a round writes
+// one pointer per thread (0 for a miss) into the round's fresh array; the
+// free kernel that follows scans the survivors of earlier rounds and the
+// fresh array, frees what the round's coin picks, and writes what it keeps
+// into regions of kRegionEntries entries, one region per free-kernel wave,
+// packed as (pool offset / 16) << 32 | live-table slot, with a count per
+// region. The next free kernel reads those regions: every round scans only
+// what is still live plus the fresh round, not every slot of every round so
+// far. Regions are double buffered; a wave writes its survivors in scan
+// order, with no atomics.
+//
+// Each wave packs the pointers it frees into full 64-lane batches in LDS
+// before calling free, so that every free call runs with a full wave (one
+// release fence per 64 frees) however sparse its entries are.
 constexpr int kFreeScanIters = 16;
+constexpr uint64_t kRegionEntries = 64ull * kFreeScanIters;
 // entries each lane loads at once, ahead of the scan steps that use them
-// (one round trip per kFreeAhead steps instead of one per step; 76 VGPRs,
-// 6 waves per SIMD; 15 % less free-kernel time per free than one at a time)
+// (one round trip per kFreeAhead steps instead of one per step)
 constexpr int kFreeAhead = 8;
 static_assert(kFreeScanIters % kFreeAhead == 0, "whole batches of loads ahead");
 constexpr int kFreeWavesPerBlock = 4;
@@ -378,17 +394,34 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// packed survivor entry <-> (pointer, live-table slot)
+__device__ inline uint64_t pack_live(uint64_t p, const char* base, uint64_t j) {
+  return (((p - (uint64_t)base) >> 4) << 32) | j;
+}
+
+// waves 0 .. n_in-1 take one input region each; waves n_in .. take
+// kRegionEntries fresh entries each (slot fresh_base + index)
 template <class A>
 __global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
-    mixed_free_kernel(A* a, uint64_t* live, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify, bool all,
+    mixed_free_kernel(A* a, const uint64_t* in_regions, const uint32_t* in_counts, uint64_t n_in,
+                      const uint64_t* fresh, uint64_t n_fresh, uint64_t fresh_base, uint64_t* out_regions,
+                      uint32_t* out_counts, uint32_t round, uint64_t seed, uint32_t verify, bool all,
                       run_counters* c) {
   __shared__ uint64_t pend_p[kFreeWavesPerBlock][128];
   __shared__ uint64_t pend_j[kFreeWavesPerBlock][128];
   const uint32_t lane = wave::lane_id();
   const uint32_t w = threadIdx.x >> 6;
   const uint64_t wave_index = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t base = wave_index * 64 * kFreeScanIters;
-  uint32_t pending = 0;  // wave-uniform count of packed entries
+  const uint64_t fresh_waves = (n_fresh + kRegionEntries - 1) / kRegionEntries;
+  if (wave_index >= n_in + fresh_waves) return;  // the whole wave
+  const char* base = a->table.memory;
+  const bool from_region = wave_index < n_in;
+  const uint64_t first = from_region ? wave_index * kRegionEntries : (wave_index - n_in) * kRegionEntries;
+  const uint64_t count = from_region ? (uint64_t)in_counts[wave_index]
+                                     : (n_fresh - first < kRegionEntries ? n_fresh - first : kRegionEntries);
+  uint64_t* out = out_regions + wave_index * kRegionEntries;
+  uint32_t kept = 0;     // wave-uniform count of survivors written
+  uint32_t pending = 0;  // wave-uniform count of packed frees
   uint64_t freed = 0, bad = 0;
   // The tags were stored by the malloc kernel, which has ended: in store
   // mode the tag is only read here, nothing is stored to the freed memory
@@ -407,23 +440,38 @@ __global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
         b = *(volatile uint64_t*)p != tag;
         a->free_released(p);
       }
-      live[j] = 0;
     }
     freed += k;
     bad += wave::popc(wave::ballot(b));
   };
+  const uint32_t iters = (uint32_t)((count + 63) / 64);
   uint64_t ahead[kFreeAhead];
-  for (int it = 0; it < kFreeScanIters; it++) {
+  for (uint32_t it = 0; it < iters; it++) {
     if (it % kFreeAhead == 0) {
 #pragma unroll
       for (int q = 0; q < kFreeAhead; q++) {
-        const uint64_t jq = base + (uint64_t)(it + q) * 64 + lane;
-        ahead[q] = jq < n_live ? live[jq] : 0;
+        const uint64_t i = (uint64_t)(it + q) * 64 + lane;
+        ahead[q] = i >= count ? 0ull : from_region ? in_regions[first + i] : fresh[first + i];
       }
     }
-    const uint64_t j = base + (uint64_t)it * 64 + lane;
-    const uint64_t p = ahead[it % kFreeAhead];
-    const bool pick = p != 0 && (all || gw::mixed_coin(seed, round, j));
+    const uint64_t i = (uint64_t)it * 64 + lane;
+    const uint64_t e = ahead[it % kFreeAhead];
+    bool live;
+    uint64_t p, j;
+    if (from_region) {
+      live = i < count;
+      p = (uint64_t)base + ((e >> 32) << 4);
+      j = e & 0xFFFFFFFFull;
+    } else {
+      live = e != 0;
+      p = e;
+      j = fresh_base + first + i;
+    }
+    const bool pick = live && (all || gw::mixed_coin(seed, round, j));
+    const bool keep = live && !pick;
+    const uint64_t km = wave::ballot(keep);
+    if (keep) out[kept + wave::popc(km & wave::lanemask_lt())] = from_region ? e : pack_live(p, base, j);
+    kept += wave::popc(km);
     const uint64_t m = wave::ballot(pick);
     if (pick) {
       const uint32_t pos = pending + wave::popc(m & wave::lanemask_lt());
@@ -449,6 +497,7 @@ __global__ void __launch_bounds__(64 * kFreeWavesPerBlock)
     }
   }
   if (pending > 0) flush(pending);
+  if (lane == 0) out_counts[wave_index] = kept;
   if (lane == 0 && freed) atom::add(&stripe_of(c)->frees, (unsigned long long)freed);
   if (lane == 0 && bad) atom::add(&stripe_of(c)->tag_errors, (unsigned long long)bad);
 }
@@ -778,12 +827,20 @@ struct impl {
     mixed_malloc_kernel<A><<<blocks_for(n, kClaimBlock), kClaimBlock, 0, h->stream>>>((A*)h->dev, h->d_live, base, n, round, seed,
                                                                        verify, h->d_counters, lat);
   }
-  static void launch_mixed_free(gal_allocator* h, uint64_t n_live, uint32_t round, uint64_t seed, uint32_t verify,
-                                bool all) {
-    if (n_live == 0) return;
-    const uint64_t waves = (n_live + 64 * kFreeScanIters - 1) / (64 * kFreeScanIters);
-    mixed_free_kernel<A><<<(unsigned)((waves + kFreeWavesPerBlock - 1) / kFreeWavesPerBlock), 64 * kFreeWavesPerBlock,
-                           0, h->stream>>>((A*)h->dev, h->d_live, n_live, round, seed, verify, all, h->d_counters);
+  // the live set (buffer cur's regions) and n_fresh fresh entries in, the
+  // survivors out into the other buffer, which becomes the live set
+  static void launch_mixed_free(gal_allocator* h, uint64_t n_fresh, uint64_t fresh_base, uint32_t round,
+                                uint64_t seed, uint32_t verify, bool all) {
+    const uint64_t n_in = h->n_regions;
+    const uint64_t waves = n_in + (n_fresh + kRegionEntries - 1) / kRegionEntries;
+    const int in = h->cur, out = h->cur ^ 1;
+    if (waves > 0)
+      mixed_free_kernel<A><<<(unsigned)((waves + kFreeWavesPerBlock - 1) / kFreeWavesPerBlock),
+                             64 * kFreeWavesPerBlock, 0, h->stream>>>(
+          (A*)h->dev, h->d_regions[in], h->d_region_counts[in], n_in, h->d_live, n_fresh, fresh_base,
+          h->d_regions[out], h->d_region_counts[out], round, seed, verify, all, h->d_counters);
+    h->cur = out;
+    h->n_regions = all ? 0 : waves;  // a drain keeps nothing
   }
   static void print(gal_allocator* h) { ((A*)h->dev)->print_info(); }
   static int fill(gal_allocator* h, gal_fill_report* out) {
@@ -1134,6 +1191,10 @@ int gal_destroy(gal_allocator* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->dev) h->ops->destroy(h->dev);
   if (h->d_live) (void)hipFree(h->d_live);
+  for (int b = 0; b < 2; b++) {
+    if (h->d_regions[b]) (void)hipFree(h->d_regions[b]);
+    if (h->d_region_counts[b]) (void)hipFree(h->d_region_counts[b]);
+  }
   if (h->d_lat) (void)hipFree(h->d_lat);
   if (h->d_counters) (void)hipFree(h->d_counters);
   for (auto& e : h->ev)
@@ -1316,21 +1377,40 @@ int gal_run_plain(gal_allocator* h, uint64_t n_threads, uint64_t size, uint64_t 
 
 int gal_mixed_reset(gal_allocator* h, uint64_t threads_per_round, uint32_t rounds, uint64_t seed) {
   if (!h || threads_per_round == 0 || rounds == 0) return GAL_ERR_INVALID;
-  // free whatever the previous run left live
-  if (h->d_live && h->rounds_done > 0) {
-    h->ops->launch_mixed_free(h, h->threads_per_round * h->rounds_done, 0, h->mixed_seed, GAL_VERIFY_STORE, true);
-    if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
-    h->rounds_done = 0;  // nothing of that run is live any more, whatever fails below
-  }
-  // every slot of the live table is written by its round's malloc kernel
-  // before any free kernel reads it, so the table needs no clearing
+  // survivors are packed as (pool offset / 16) << 32 | live-table slot
   const uint64_t need = threads_per_round * rounds;
-  if (need > h->live_cap) {
+  if (need >= (1ull << 32) || h->info.num_segments * h->info.segment_size > (16ull << 32)) return GAL_ERR_INVALID;
+  // free whatever the previous run left live
+  if (h->d_live && h->n_regions > 0) {
+    h->ops->launch_mixed_free(h, 0, 0, 0, h->mixed_seed, GAL_VERIFY_STORE, true);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  }
+  h->rounds_done = 0;  // nothing of that run is live any more, whatever fails below
+  h->n_regions = 0;
+  // the fresh array is written by each round's malloc kernel before its free
+  // kernel reads it, and a region's entries below its count are written
+  // before any kernel reads them: nothing needs clearing
+  if (threads_per_round > h->live_cap) {
     if (h->d_live) (void)hipFree(h->d_live);
     h->d_live = nullptr;
     h->live_cap = 0;
-    if (hipMalloc(&h->d_live, need * 8) != hipSuccess) return GAL_ERR_OOM;
-    h->live_cap = need;
+    if (hipMalloc(&h->d_live, threads_per_round * 8) != hipSuccess) return GAL_ERR_OOM;
+    h->live_cap = threads_per_round;
+  }
+  const uint64_t regions = (uint64_t)rounds * ((threads_per_round + kRegionEntries - 1) / kRegionEntries);
+  if (regions > h->region_cap) {
+    for (int b = 0; b < 2; b++) {
+      if (h->d_regions[b]) (void)hipFree(h->d_regions[b]);
+      if (h->d_region_counts[b]) (void)hipFree(h->d_region_counts[b]);
+      h->d_regions[b] = nullptr;
+      h->d_region_counts[b] = nullptr;
+    }
+    h->region_cap = 0;
+    for (int b = 0; b < 2; b++)
+      if (hipMalloc(&h->d_regions[b], regions * kRegionEntries * 8) != hipSuccess ||
+          hipMalloc(&h->d_region_counts[b], regions * 4) != hipSuccess)
+        return GAL_ERR_OOM;
+    h->region_cap = regions;
   }
   const uint64_t lat_need = (need + 63) / 64 + 1;
   if (lat_need > h->lat_cap) {
@@ -1358,7 +1438,8 @@ int gal_mixed_round(gal_allocator* h, uint32_t round, uint32_t verify, gal_run_r
                                 h->d_lat);
     t.end(0);
     if (!t.begin()) return GAL_ERR_INVALID;
-    h->ops->launch_mixed_free(h, (uint64_t)(round + 1) * h->threads_per_round, round, seed, verify, false);
+    h->ops->launch_mixed_free(h, h->threads_per_round, (uint64_t)round * h->threads_per_round, round, seed, verify,
+                              false);
     t.end(1);
     return GAL_OK;
   });
@@ -1372,11 +1453,10 @@ int gal_mixed_round(gal_allocator* h, uint32_t round, uint32_t verify, gal_run_r
 int gal_mixed_drain(gal_allocator* h, uint32_t verify, gal_run_result* out) {
   if (!h || !h->d_live || verify > GAL_VERIFY_ATOMIC) return GAL_ERR_INVALID;
   const uint64_t seed = h->mixed_seed;
-  const uint64_t n_live = (uint64_t)h->rounds_done * h->threads_per_round;
   int rc = timed_run(h, out, [&](launch_timer& t) {
-    if (n_live == 0) return GAL_OK;
+    if (h->n_regions == 0) return GAL_OK;
     if (!t.begin()) return GAL_ERR_INVALID;
-    h->ops->launch_mixed_free(h, n_live, 0, seed, verify, true);
+    h->ops->launch_mixed_free(h, 0, 0, 0, seed, verify, true);
     t.end(1);
     return GAL_OK;
   });
@@ -1387,20 +1467,26 @@ int gal_mixed_drain(gal_allocator* h, uint32_t verify, gal_run_result* out) {
 int gal_mixed_live(gal_allocator* h, uint64_t* offsets, uint64_t* sizes, uint64_t* slots, uint64_t cap,
                    uint64_t* count) {
   if (!h || !count) return GAL_ERR_INVALID;
-  const uint64_t n_live = (uint64_t)h->rounds_done * h->threads_per_round;
-  std::vector<uint64_t> ptrs(n_live);
   if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
-  if (n_live && hipMemcpy(ptrs.data(), h->d_live, n_live * 8, hipMemcpyDeviceToHost) != hipSuccess)
+  const uint64_t n = h->n_regions;
+  std::vector<uint32_t> counts(n);
+  std::vector<uint64_t> entries(n * kRegionEntries);
+  if (n && (hipMemcpy(counts.data(), h->d_region_counts[h->cur], n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(entries.data(), h->d_regions[h->cur], n * kRegionEntries * 8, hipMemcpyDeviceToHost) !=
+                hipSuccess))
     return GAL_ERR_HIP;
   uint64_t k = 0;
-  for (uint64_t j = 0; j < n_live; j++) {
-    if (!ptrs[j]) continue;
-    if (k < cap) {
-      if (offsets) offsets[k] = ptrs[j] - h->info.pool_device_ptr;
-      if (sizes) sizes[k] = gw::mixed_size(h->mixed_seed, j / h->threads_per_round, j % h->threads_per_round);
-      if (slots) slots[k] = j;
+  for (uint64_t r = 0; r < n; r++) {
+    for (uint32_t i = 0; i < counts[r] && i < kRegionEntries; i++) {
+      const uint64_t e = entries[r * kRegionEntries + i];
+      const uint64_t j = e & 0xFFFFFFFFull;
+      if (k < cap) {
+        if (offsets) offsets[k] = (e >> 32) << 4;
+        if (sizes) sizes[k] = gw::mixed_size(h->mixed_seed, j / h->threads_per_round, j % h->threads_per_round);
+        if (slots) slots[k] = j;
+      }
+      k++;
     }
-    k++;
   }
   *count = k;
   return (offsets && k > cap) ? GAL_ERR_BUFFER : GAL_OK;
@@ -1418,11 +1504,11 @@ int gal_run_mixed(gal_allocator* h, uint64_t threads_per_round, uint32_t rounds,
       h->ops->launch_mixed_malloc(h, (uint64_t)r * threads_per_round, threads_per_round, r, seed, verify, h->d_lat);
       t.end(0);
       if (!t.begin()) return GAL_ERR_INVALID;
-      h->ops->launch_mixed_free(h, (uint64_t)(r + 1) * threads_per_round, r, seed, verify, false);
+      h->ops->launch_mixed_free(h, threads_per_round, (uint64_t)r * threads_per_round, r, seed, verify, false);
       t.end(1);
     }
     if (!t.begin()) return GAL_ERR_INVALID;
-    h->ops->launch_mixed_free(h, (uint64_t)rounds * threads_per_round, 0, seed, verify, true);
+    h->ops->launch_mixed_free(h, 0, 0, 0, seed, verify, true);
     t.end(1);
     return GAL_OK;
   });
