@@ -439,7 +439,15 @@ struct Gallatin {
     const bool is_leader = (int)me == leader;
 
     uint32_t index, entry;
-    const uint32_t lead_hint = wave::shfl(hint, leader);
+    // Under load the hint (the CU id for malloc) is mixed with the wave's
+    // index in its block: the waves of one block run on one CU, and with the
+    // CU id alone all of them would start every class's claim on the same
+    // slot. A single lane keeps the hint as given.
+    const bool multi = (looking & (looking - 1)) != 0;
+    const uint32_t wave_in_block = (uint32_t)(threadIdx.x >> 6);
+    const uint32_t lead_hint = spread && multi && wave_in_block != 0
+                                   ? wave::shfl(hint, leader) + wave_in_block * 0x9E3779B1u
+                                   : wave::shfl(hint, leader);
     const uint32_t usable = usable_slots[tree];
     // Under load a group's first pass starts at slot hint % usable; an empty
     // slot there is filled unless the class is out of memory. Otherwise (and
