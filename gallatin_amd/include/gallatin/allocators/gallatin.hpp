@@ -455,6 +455,7 @@ struct Gallatin {
     bool placed = false;
     bool dry = false;    // the leader found the class out of memory in this pass
     bool spare = false;  // the leader saw free segments in this pass (a refill can get a block)
+    bool look_refill = false;  // the group refills the empty slot its first look landed on
     //
     // A class whose usable slots are its pinned slots (the two smallest
     // classes) takes the same first look at slot hint % pinned, where the
@@ -491,6 +492,7 @@ struct Gallatin {
           index = si;
           entry = 0u;
           placed = true;
+          look_refill = !dry;
         }
       } else if (se == 0u && tree_dry && all_empty) {
         dry = true;
@@ -502,6 +504,7 @@ struct Gallatin {
     placed = wave::shfl((uint32_t)placed, leader) != 0u;
     dry = wave::shfl((uint32_t)dry, leader) != 0u;
     spare = wave::shfl((uint32_t)spare, leader) != 0u;
+    look_refill = wave::shfl((uint32_t)look_refill, leader) != 0u;
     if (placed) {
       index = wave::shfl(index, leader);
       entry = wave::shfl(entry, leader);
@@ -615,7 +618,10 @@ struct Gallatin {
     }
     if (wave::ballot(busy) & looking) wave::backoff(8);
     waited = busy;
-    if (empty && wave::shfl((uint32_t)refilled, leader) == 0u) {  // class out of memory
+    // A refill that finds no block is the class out of memory, unless the
+    // slot was the first look's: blocks there can still be in the pinned
+    // slots, which the next pass walks (it takes no first look).
+    if (empty && wave::shfl((uint32_t)refilled, leader) == 0u && !look_refill) {  // class out of memory
       done = true;
       oom = true;
       result = utils::kFail;
