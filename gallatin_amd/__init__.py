@@ -4,7 +4,11 @@ Host mirror of the reference's host interface (include/gallatin/allocators/
 gallatin.cuh:324-421, 643-668, 1855-1897 and global_allocator.cuh:48-97),
 backed by the HIP library gallatin_amd/lib/libgallatin_amd.so. The device
 interface (Gallatin<SEG,MIN,MAX>::malloc/free, global_malloc/global_free) is
-the C++ header set in gallatin_amd/include/gallatin/.
+the C++ header set in gallatin_amd/include/gallatin/. The workloads that
+drive an allocator (run_plain, the mixed rounds, churn, batches, the serving
+loop, the slab table, the global-API kernels) come from the separate library
+gallatin_amd/lib/libgallatin_workloads.so, each through a workload object
+built on the allocator's handle.
 """
 import ctypes
 import weakref
@@ -28,6 +32,14 @@ DEFAULT_MIN = 16
 DEFAULT_MAX = 4096
 
 _lib = _capi.lib
+_wl = _capi.wl
+
+
+def _new_workload(h):
+    """A workload object (libgallatin_workloads.so) on allocator handle h."""
+    w = ctypes.c_void_p()
+    _capi.check("gal_workload_create", _wl.gal_workload_create(h, ctypes.byref(w)))
+    return w
 
 
 def supported_triples():
@@ -72,6 +84,18 @@ class Gallatin:
         self.info = info
         self._owned = owned
         self._children = []  # weak references to batches, serving loops, slab tables made on it
+        self._w = None  # this handle's workload object, made on first use
+
+    def _workload(self):
+        if self._w is None:
+            self._w = _new_workload(self._h)
+        return self._w
+
+    def _drop_workload(self):
+        """Destroy the workload object (and its hold on the handle) unless a
+        batch, serving loop or slab table still uses it."""
+        if self._w is not None and _wl.gal_workload_destroy(self._w) == _capi.GAL_OK:
+            self._w = None
 
     def _adopt(self, child):
         self._children = [c for c in self._children if c() is not None]
@@ -127,6 +151,9 @@ class Gallatin:
             if child is not None:
                 child._release()
         self._children = []
+        if self._w is not None:
+            _capi.check("gal_workload_destroy", _wl.gal_workload_destroy(self._w))
+            self._w = None
         if self._h is not None and self._owned:
             _capi.check("gal_destroy", _lib.gal_destroy(self._h))
         self._h = None
@@ -242,25 +269,25 @@ class Gallatin:
 
     def run_plain(self, n_threads, size, seed, verify=GAL_VERIFY_STORE):
         r = _capi.GalRunResult()
-        _capi.check("gal_run_plain", _lib.gal_run_plain(self._h, n_threads, size, seed, verify, ctypes.byref(r)))
+        _capi.check("gal_run_plain", _wl.gal_run_plain(self._workload(), n_threads, size, seed, verify, ctypes.byref(r)))
         return r.as_dict()
 
     def mixed_reset(self, threads_per_round, rounds, seed):
-        _capi.check("gal_mixed_reset", _lib.gal_mixed_reset(self._h, threads_per_round, rounds, seed))
+        _capi.check("gal_mixed_reset", _wl.gal_mixed_reset(self._workload(), threads_per_round, rounds, seed))
 
     def mixed_round(self, rnd, verify=GAL_VERIFY_STORE):
         r = _capi.GalRunResult()
-        _capi.check("gal_mixed_round", _lib.gal_mixed_round(self._h, rnd, verify, ctypes.byref(r)))
+        _capi.check("gal_mixed_round", _wl.gal_mixed_round(self._workload(), rnd, verify, ctypes.byref(r)))
         return r.as_dict()
 
     def mixed_drain(self, verify=GAL_VERIFY_STORE):
         r = _capi.GalRunResult()
-        _capi.check("gal_mixed_drain", _lib.gal_mixed_drain(self._h, verify, ctypes.byref(r)))
+        _capi.check("gal_mixed_drain", _wl.gal_mixed_drain(self._workload(), verify, ctypes.byref(r)))
         return r.as_dict()
 
     def mixed_live(self):
         n = ctypes.c_uint64()
-        _capi.check("gal_mixed_live", _lib.gal_mixed_live(self._h, None, None, None, 0, ctypes.byref(n)))
+        _capi.check("gal_mixed_live", _wl.gal_mixed_live(self._workload(), None, None, None, 0, ctypes.byref(n)))
         cnt = n.value
         offs = np.zeros(max(1, cnt), dtype=np.uint64)
         sizes = np.zeros(max(1, cnt), dtype=np.uint64)
@@ -268,8 +295,8 @@ class Gallatin:
         p = ctypes.POINTER(ctypes.c_uint64)
         _capi.check(
             "gal_mixed_live",
-            _lib.gal_mixed_live(
-                self._h,
+            _wl.gal_mixed_live(
+                self._workload(),
                 offs.ctypes.data_as(p),
                 sizes.ctypes.data_as(p),
                 slots.ctypes.data_as(p),
@@ -282,19 +309,19 @@ class Gallatin:
     def run_mixed(self, threads_per_round, rounds, seed, verify=GAL_VERIFY_STORE):
         r = _capi.GalRunResult()
         _capi.check(
-            "gal_run_mixed", _lib.gal_run_mixed(self._h, threads_per_round, rounds, seed, verify, ctypes.byref(r))
+            "gal_run_mixed", _wl.gal_run_mixed(self._workload(), threads_per_round, rounds, seed, verify, ctypes.byref(r))
         )
         return r.as_dict()
 
     def latency_ns(self):
         """Per-wave malloc latencies (ns) of the last mixed run."""
         n, tick = ctypes.c_uint64(), ctypes.c_double()
-        _capi.check("gal_latency_samples", _lib.gal_latency_samples(self._h, None, 0, ctypes.byref(n), ctypes.byref(tick)))
+        _capi.check("gal_latency_samples", _wl.gal_latency_samples(self._workload(), None, 0, ctypes.byref(n), ctypes.byref(tick)))
         buf = np.zeros(max(1, n.value), dtype=np.uint32)
         _capi.check(
             "gal_latency_samples",
-            _lib.gal_latency_samples(
-                self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n.value, ctypes.byref(n), None
+            _wl.gal_latency_samples(
+                self._workload(), buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n.value, ctypes.byref(n), None
             ),
         )
         return buf[: n.value].astype(np.float64) * tick.value
@@ -314,7 +341,7 @@ class Gallatin:
         r = _capi.GalRunResult()
         _capi.check(
             "gal_churn",
-            _lib.gal_churn(self._h, arr, n, seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(r)),
+            _wl.gal_churn(self._workload(), arr, n, seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(r)),
         )
         return out[:n], r.as_dict()
 
@@ -339,11 +366,22 @@ class Gallatin:
 
 
 # ---- global allocator (reference global_allocator.cuh:41-97)
+_global_views = weakref.WeakSet()  # Gallatin views handed out by global_handles()
+
+
+def _drop_view_workloads():
+    # a view's idle workload object holds the global handle; the free calls
+    # below would refuse it (GAL_ERR_STATE), so it goes first
+    for v in list(_global_views):
+        v._drop_workload()
+
+
 def init_global_allocator(num_bytes, seed, print_info=True):
     _capi.check("gal_init_global_allocator", _lib.gal_init_global_allocator(num_bytes, seed, int(bool(print_info))))
 
 
 def free_global_allocator():
+    _drop_view_workloads()
     _capi.check("gal_free_global_allocator", _lib.gal_free_global_allocator())
 
 
@@ -366,6 +404,7 @@ def init_global_allocator_host(num_bytes, seed, print_info=True):
 
 
 def free_global_allocator_host():
+    _drop_view_workloads()
     _capi.check("gal_free_global_allocator_host", _lib.gal_free_global_allocator_host())
 
 
@@ -379,6 +418,7 @@ def init_global_allocator_combined(num_bytes, host_bytes, seed, print_info=True)
 
 
 def free_global_allocator_combined():
+    _drop_view_workloads()
     _capi.check("gal_free_global_allocator_combined", _lib.gal_free_global_allocator_combined())
 
 
@@ -397,14 +437,16 @@ def global_handles():
             continue
         info = _capi.GalInfo()
         _capi.check("gal_get_info", _lib.gal_get_info(x, ctypes.byref(info)))
-        out.append(Gallatin(x, info, owned=False))
+        view = Gallatin(x, info, owned=False)
+        _global_views.add(view)
+        out.append(view)
     return tuple(out)
 
 
 def global_fused(n_threads, size):
     """global_malloc_fused / global_free_fused from n threads: (on_device, on_host, misses, tag_errors)."""
     a, b, c, d = (ctypes.c_uint64() for _ in range(4))
-    _capi.check("gal_global_fused", _lib.gal_global_fused(n_threads, size, ctypes.byref(a), ctypes.byref(b),
+    _capi.check("gal_global_fused", _wl.gal_global_fused(n_threads, size, ctypes.byref(a), ctypes.byref(b),
                                                           ctypes.byref(c), ctypes.byref(d)))
     return a.value, b.value, c.value, d.value
 
@@ -412,14 +454,14 @@ def global_fused(n_threads, size):
 def global_poison(size):
     """Guard-check results of the reference's poison sequence for one size (gal_global_poison)."""
     codes = (ctypes.c_int32 * 7)()
-    _capi.check("gal_global_poison", _lib.gal_global_poison(size, codes, 7))
+    _capi.check("gal_global_poison", _wl.gal_global_poison(size, codes, 7))
     return list(codes)
 
 
 def global_plain(n_threads, size, seed, verify=GAL_VERIFY_STORE):
     """n_threads threads each global_malloc(size), tag, check, global_free (timed)."""
     r = _capi.GalRunResult()
-    _capi.check("gal_global_plain", _lib.gal_global_plain(n_threads, size, seed, verify, ctypes.byref(r)))
+    _capi.check("gal_global_plain", _wl.gal_global_plain(n_threads, size, seed, verify, ctypes.byref(r)))
     return r.as_dict()
 
 
@@ -435,7 +477,7 @@ def global_replay(ops):
 
 def global_smoke(n_threads, size):
     m, t = ctypes.c_uint64(), ctypes.c_uint64()
-    _capi.check("gal_global_smoke", _lib.gal_global_smoke(n_threads, size, ctypes.byref(m), ctypes.byref(t)))
+    _capi.check("gal_global_smoke", _wl.gal_global_smoke(n_threads, size, ctypes.byref(m), ctypes.byref(t)))
     return m.value, t.value
 
 
@@ -447,11 +489,15 @@ FORM_GRAPH = _capi.GAL_FORM_GRAPH
 
 class Batch:
     """Requests served together: binned by class on the device, then one
-    persistent launch (see gal_batch_* in include/gallatin_amd.h)."""
+    persistent launch (see gal_batch_* in include/gallatin_workloads.h)."""
 
     def __init__(self, allocator, capacity):
+        self._w = _new_workload(allocator._h)
         h = ctypes.c_void_p()
-        _capi.check("gal_batch_create", _lib.gal_batch_create(allocator._h, capacity, ctypes.byref(h)))
+        rc = _wl.gal_batch_create(self._w, capacity, ctypes.byref(h))
+        if rc != _capi.GAL_OK:
+            _wl.gal_workload_destroy(self._w)
+            _capi.check("gal_batch_create", rc)
         self._b = h
         self.allocator = allocator
         self.capacity = capacity
@@ -460,8 +506,11 @@ class Batch:
 
     def close(self):
         if self._b:
-            _capi.check("gal_batch_destroy", _lib.gal_batch_destroy(self._b))
+            _capi.check("gal_batch_destroy", _wl.gal_batch_destroy(self._b))
             self._b = None
+        if self._w:
+            _capi.check("gal_workload_destroy", _wl.gal_workload_destroy(self._w))
+            self._w = None
 
     _release = close
 
@@ -472,42 +521,42 @@ class Batch:
         self.close()
 
     def generate(self, n, seed, size_classes=9):
-        _capi.check("gal_batch_generate", _lib.gal_batch_generate(self._b, n, seed, size_classes))
+        _capi.check("gal_batch_generate", _wl.gal_batch_generate(self._b, n, seed, size_classes))
         self.n = n
 
     def set_sizes(self, sizes, seed=0):
         arr = np.ascontiguousarray(sizes, dtype=np.uint64)
         p = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
-        _capi.check("gal_batch_set_sizes", _lib.gal_batch_set_sizes(self._b, p, len(arr), seed))
+        _capi.check("gal_batch_set_sizes", _wl.gal_batch_set_sizes(self._b, p, len(arr), seed))
         self.n = len(arr)
 
     def run(self, form, width=64, replays=1, verify=GAL_VERIFY_STORE, run=1):
         r = _capi.GalRunResult()
         _capi.check(
-            "gal_batch_run", _lib.gal_batch_run(self._b, form, width, run, replays, verify, ctypes.byref(r))
+            "gal_batch_run", _wl.gal_batch_run(self._b, form, width, run, replays, verify, ctypes.byref(r))
         )
         return r.as_dict()
 
     def malloc(self, width=64, run=1):
         r = _capi.GalRunResult()
-        _capi.check("gal_batch_malloc", _lib.gal_batch_malloc(self._b, width, run, ctypes.byref(r)))
+        _capi.check("gal_batch_malloc", _wl.gal_batch_malloc(self._b, width, run, ctypes.byref(r)))
         return r.as_dict()
 
     def free(self):
         r = _capi.GalRunResult()
-        _capi.check("gal_batch_free", _lib.gal_batch_free(self._b, ctypes.byref(r)))
+        _capi.check("gal_batch_free", _wl.gal_batch_free(self._b, ctypes.byref(r)))
         return r.as_dict()
 
     def offsets(self):
         out = np.zeros(max(1, self.n), dtype=np.uint64)
         p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
-        _capi.check("gal_batch_offsets", _lib.gal_batch_offsets(self._b, p, self.n))
+        _capi.check("gal_batch_offsets", _wl.gal_batch_offsets(self._b, p, self.n))
         return out[: self.n]
 
     def order(self):
         out = np.zeros(max(1, self.n), dtype=np.uint32)
         p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
-        _capi.check("gal_batch_order", _lib.gal_batch_order(self._b, p, self.n))
+        _capi.check("gal_batch_order", _wl.gal_batch_order(self._b, p, self.n))
         return out[: self.n]
 
 
@@ -534,18 +583,22 @@ def generate_requests(n, rate_per_s, median_pages=1024.0, sigma=1.0, max_pages=3
     w = _capi.GalServeWorkload(n, float(rate_per_s), float(median_pages), float(sigma), max_pages, page_bytes,
                                size_classes, 0, seed)
     out = np.zeros(n, dtype=REQUEST_DTYPE)
-    _capi.check("gal_serve_generate", _lib.gal_serve_generate(ctypes.byref(w), out.ctypes.data_as(ctypes.c_void_p)))
+    _capi.check("gal_serve_generate", _wl.gal_serve_generate(ctypes.byref(w), out.ctypes.data_as(ctypes.c_void_p)))
     return out
 
 
 class Serving:
     """A device-resident serving loop over one allocator (gal_serve_* in
-    include/gallatin_amd.h): sequences arrive, are admitted by free pool
+    include/gallatin_workloads.h): sequences arrive, are admitted by free pool
     bytes, grow one page per step and free their pages when done."""
 
     def __init__(self, allocator, capacity):
+        self._w = _new_workload(allocator._h)
         h = ctypes.c_void_p()
-        _capi.check("gal_serve_create", _lib.gal_serve_create(allocator._h, capacity, ctypes.byref(h)))
+        rc = _wl.gal_serve_create(self._w, capacity, ctypes.byref(h))
+        if rc != _capi.GAL_OK:
+            _wl.gal_workload_destroy(self._w)
+            _capi.check("gal_serve_create", rc)
         self._s = h
         self.allocator = allocator
         self.n = 0
@@ -553,8 +606,11 @@ class Serving:
 
     def close(self):
         if self._s:
-            _capi.check("gal_serve_destroy", _lib.gal_serve_destroy(self._s))
+            _capi.check("gal_serve_destroy", _wl.gal_serve_destroy(self._s))
             self._s = None
+        if self._w:
+            _capi.check("gal_workload_destroy", _wl.gal_workload_destroy(self._w))
+            self._w = None
 
     _release = close
 
@@ -566,13 +622,13 @@ class Serving:
 
     def load(self, requests):
         arr = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
-        _capi.check("gal_serve_load", _lib.gal_serve_load(self._s, arr.ctypes.data_as(ctypes.c_void_p), len(arr)))
+        _capi.check("gal_serve_load", _wl.gal_serve_load(self._s, arr.ctypes.data_as(ctypes.c_void_p), len(arr)))
         self.n = len(arr)
 
     def run(self, slots=16384, run=1, headroom=SERVE_DEFAULT_HEADROOM, deadline_s=60.0, window_s=0.0):
         r = _capi.GalServeResult()
         _capi.check("gal_serve_run",
-                    _lib.gal_serve_run(self._s, slots, run, headroom, deadline_s, window_s, ctypes.byref(r)))
+                    _wl.gal_serve_run(self._s, slots, run, headroom, deadline_s, window_s, ctypes.byref(r)))
         return r.as_dict()
 
     def results(self):
@@ -583,7 +639,7 @@ class Serving:
         p = ctypes.POINTER(ctypes.c_uint64)
         _capi.check(
             "gal_serve_results",
-            _lib.gal_serve_results(self._s, out[0].ctypes.data_as(p), out[1].ctypes.data_as(p),
+            _wl.gal_serve_results(self._s, out[0].ctypes.data_as(p), out[1].ctypes.data_as(p),
                                    out[2].ctypes.data_as(p), self.n),
         )
         return tuple(a[: self.n] for a in out)
@@ -592,11 +648,15 @@ class Serving:
 # ---- slab hash client (reference data_structs/chaining_table.cuh)
 class SlabTable:
     """Chained slab hash table whose 120-byte slabs come from the allocator
-    (gal_slab_* in include/gallatin_amd.h)."""
+    (gal_slab_* in include/gallatin_workloads.h)."""
 
     def __init__(self, allocator, buckets, seed=123456):
+        self._w = _new_workload(allocator._h)
         h = ctypes.c_void_p()
-        _capi.check("gal_slab_create", _lib.gal_slab_create(allocator._h, buckets, seed, ctypes.byref(h)))
+        rc = _wl.gal_slab_create(self._w, buckets, seed, ctypes.byref(h))
+        if rc != _capi.GAL_OK:
+            _wl.gal_workload_destroy(self._w)
+            _capi.check("gal_slab_create", rc)
         self._t = h
         self.allocator = allocator
         self.buckets = buckets
@@ -604,19 +664,22 @@ class SlabTable:
 
     def insert(self, n, key_seed):
         r = _capi.GalSlabResult()
-        _capi.check("gal_slab_insert", _lib.gal_slab_insert(self._t, n, key_seed, ctypes.byref(r)))
+        _capi.check("gal_slab_insert", _wl.gal_slab_insert(self._t, n, key_seed, ctypes.byref(r)))
         return r.as_dict()
 
     def query(self, n, key_seed):
         r = _capi.GalSlabResult()
-        _capi.check("gal_slab_query", _lib.gal_slab_query(self._t, n, key_seed, ctypes.byref(r)))
+        _capi.check("gal_slab_query", _wl.gal_slab_query(self._t, n, key_seed, ctypes.byref(r)))
         return r.as_dict()
 
     def destroy(self):
         freed = ctypes.c_uint64()
         if self._t:
-            _capi.check("gal_slab_destroy", _lib.gal_slab_destroy(self._t, ctypes.byref(freed)))
+            _capi.check("gal_slab_destroy", _wl.gal_slab_destroy(self._t, ctypes.byref(freed)))
             self._t = None
+        if self._w:
+            _capi.check("gal_workload_destroy", _wl.gal_workload_destroy(self._w))
+            self._w = None
         return freed.value
 
     def _release(self):
@@ -637,5 +700,5 @@ ATOMIC_WAVE_HOT = 1
 def atomic_rate(mode, threads, iters, addresses=1):
     """Returned agent-scope 32-bit atomic adds per second (see gal_atomic_rate)."""
     out = ctypes.c_double()
-    _capi.check("gal_atomic_rate", _lib.gal_atomic_rate(mode, threads, iters, addresses, ctypes.byref(out)))
+    _capi.check("gal_atomic_rate", _wl.gal_atomic_rate(mode, threads, iters, addresses, ctypes.byref(out)))
     return out.value

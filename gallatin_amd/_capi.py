@@ -1,7 +1,9 @@
-"""ctypes binding of libgallatin_amd.so (the C-ABI in include/gallatin_amd.h).
+"""ctypes bindings of libgallatin_amd.so (the allocator's C-ABI,
+include/gallatin_amd.h) and libgallatin_workloads.so (the benchmark and test
+workloads on top of it, include/gallatin_workloads.h).
 
-The library is the product: there is no CPU fallback. If it has not been
-built, importing this module raises.
+The libraries are the product: there is no CPU fallback. If they have not
+been built, importing this module raises.
 """
 import ctypes
 import os
@@ -9,6 +11,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libgallatin_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gallatin_amd.h")
+WL_PATH = os.path.join(_HERE, "lib", "libgallatin_workloads.so")
+WL_HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gallatin_workloads.h")
 
 GAL_OK = 0
 GAL_ERR_INVALID = 1
@@ -184,8 +188,6 @@ _SIGNATURES = {
     "gal_init_global_allocator": (ctypes.c_int, [u64, u64, ctypes.c_int]),
     "gal_free_global_allocator": (ctypes.c_int, []),
     "gal_print_global_stats": (ctypes.c_int, []),
-    "gal_global_smoke": (ctypes.c_int, [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
-    "gal_global_plain": (ctypes.c_int, [u64, u64, u64, u32, ctypes.POINTER(GalRunResult)]),
     "gal_global_replay": (ctypes.c_int, [ctypes.POINTER(GalOp), u64, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
     "gal_create_host": (ctypes.c_int, [u64, u64, u64, u64, u64, ctypes.POINTER(vp)]),
     "gal_allocated_size": (ctypes.c_int, [u64, u64, u64, u64, ctypes.POINTER(u64)]),
@@ -196,6 +198,20 @@ _SIGNATURES = {
     "gal_free_global_allocator_combined": (ctypes.c_int, []),
     "gal_print_global_stats_combined": (ctypes.c_int, []),
     "gal_global_handles": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+    "gal_read_pool": (ctypes.c_int, [vp, u64, vp, u64]),
+    "gal_free_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "gal_gather_limit": (ctypes.c_int, [ctypes.POINTER(u64), u64, ctypes.POINTER(u64)]),
+    "gal_largest_run": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "gal_stream": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    "gal_retain": (ctypes.c_int, [vp]),
+    "gal_release": (ctypes.c_int, [vp]),
+}
+
+_WL_SIGNATURES = {
+    "gal_workload_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    "gal_workload_destroy": (ctypes.c_int, [vp]),
+    "gal_global_smoke": (ctypes.c_int, [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "gal_global_plain": (ctypes.c_int, [u64, u64, u64, u32, ctypes.POINTER(GalRunResult)]),
     "gal_global_fused": (
         ctypes.c_int,
         [u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)],
@@ -210,7 +226,6 @@ _SIGNATURES = {
         [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64), u64, ctypes.POINTER(u64)],
     ),
     "gal_run_mixed": (ctypes.c_int, [vp, u64, u32, u64, u32, ctypes.POINTER(GalRunResult)]),
-    "gal_read_pool": (ctypes.c_int, [vp, u64, vp, u64]),
     "gal_atomic_rate": (ctypes.c_int, [u32, u64, u32, u32, ctypes.POINTER(ctypes.c_double)]),
     "gal_batch_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
     "gal_batch_destroy": (ctypes.c_int, [vp]),
@@ -225,14 +240,11 @@ _SIGNATURES = {
         ctypes.c_int,
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],
     ),
-    "gal_free_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "gal_slab_create": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(vp)]),
     "gal_slab_insert": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(GalSlabResult)]),
     "gal_slab_query": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(GalSlabResult)]),
     "gal_slab_destroy": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "gal_churn": (ctypes.c_int, [vp, ctypes.POINTER(GalOp), u64, u64, ctypes.POINTER(u64), ctypes.POINTER(GalRunResult)]),
-    "gal_gather_limit": (ctypes.c_int, [ctypes.POINTER(u64), u64, ctypes.POINTER(u64)]),
-    "gal_largest_run": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "gal_serve_generate": (ctypes.c_int, [ctypes.POINTER(GalServeWorkload), vp]),
     "gal_serve_create": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
     "gal_serve_destroy": (ctypes.c_int, [vp]),
@@ -254,14 +266,14 @@ class GallatinError(RuntimeError):
         self.code = code
 
 
-def load():
-    if not os.path.exists(LIB_PATH):
+def load(path=LIB_PATH, signatures=None):
+    if not os.path.exists(path):
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "or `make -C gallatin_amd/csrc`"
         )
-    lib = ctypes.CDLL(LIB_PATH)
-    for name, (res, args) in _SIGNATURES.items():
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in (_SIGNATURES if signatures is None else signatures).items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
@@ -274,3 +286,5 @@ def check(func, code):
 
 
 lib = load()
+# after the allocator library, which it links against (the loader maps that once)
+wl = load(WL_PATH, _WL_SIGNATURES)

@@ -1,4 +1,4 @@
-// batch.inl — batched service of uneven request sizes (part of capi.hip).
+// batch.inl — batched service of uneven request sizes (part of workloads.hip).
 //
 // A batch is an array of request sizes. It is served in three steps, all on
 // the device: the requests are binned by the class they claim from (a stable

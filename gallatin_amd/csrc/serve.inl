@@ -1,4 +1,4 @@
-// serve.inl — the serving loop (part of capi.hip).
+// serve.inl — the serving loop (part of workloads.hip).
 //
 // One persistent launch serves a stream of sequence requests. Every lane of
 // the grid is a sequence slot with a small state machine:

@@ -1,4 +1,4 @@
-// slab.inl — the slab hash client (part of capi.hip): a chaining_table of
+// slab.inl — the slab hash client (part of workloads.hip): a chaining_table of
 // 64-bit keys and values, 7 per slab, whose slabs come from the handle's
 // allocator (reference chaining_table.cuh, tests/src/chaining_test.cu).
 

@@ -438,7 +438,7 @@ struct Gallatin {
     const int last = wave::last_lane(group);
     const bool is_leader = (int)me == leader;
 
-    uint32_t index, entry;
+    uint32_t index = 0u, entry = 0u;
     // Under load the hint (the CU id for malloc) is mixed with the wave's
     // index in its block: the waves of one block run on one CU, and with the
     // CU id alone all of them would start every class's claim on the same
