@@ -45,12 +45,14 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "",
-    "commit": "",
-    "traffic": {"mixed_malloc_kernel": None, "mixed_free_kernel": None},
-    "atomic_requests": {"mixed_malloc_kernel": None, "mixed_free_kernel": None},
+    "sources_sha": "b59646539728bbe3",
+    "commit": "255d759",
+    "traffic": {"mixed_malloc_kernel": 200416752, "mixed_free_kernel": 190670231},
+    "atomic_requests": {"mixed_malloc_kernel": 280127, "mixed_free_kernel": 534838},
 }
-KERNEL_SOURCES = ("gallatin_amd/csrc/*.hip", "gallatin_amd/csrc/*.inl", "gallatin_amd/include/gallatin/*.hpp",
+# the sources the mixed workload's kernels are compiled from: their kernels
+# (workloads.hip) and the device allocator (headers)
+KERNEL_SOURCES = ("gallatin_amd/csrc/workloads.hip", "gallatin_amd/include/gallatin/*.hpp",
                   "gallatin_amd/include/gallatin/allocators/*.hpp")
 DEFAULT_WORKLOAD = (8, 22, 10)  # pool GiB, log2 threads per round, rounds
 
