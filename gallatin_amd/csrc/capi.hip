@@ -84,8 +84,10 @@ __global__ void replay_kernel(A* a, const gal_op* ops, uint64_t n, uint64_t* res
 
 // One wavefront runs the group ops in order: the lanes of op i's mask reach
 // the malloc (or free) call together, with exactly that exec mask. Claims
-// use the reference's pinned-slot walk (no overflow-slot spreading), so the
-// results depend on the op stream only. Every lane writes its own result
+// of GAL_OP_MALLOC use the reference's pinned-slot walk (no overflow-slot
+// spreading); GAL_OP_MALLOC_SPREAD takes malloc's first look under load
+// (the block has one wave, so the hint is the op's slot unmixed). Either way
+// the results depend on the op stream only. Every lane writes its own result
 // entry (GAL_NULL_OFFSET for lanes outside the op); a free reads the result
 // of an earlier op of this wave, written by any of its lanes, after the
 // workgroup-scope fence that ends each op.
@@ -108,21 +110,23 @@ __global__ void __launch_bounds__(64) group_replay_kernel(A* a, const gal_group_
     const uint32_t kind = op->kind;
     const bool in = (lanes >> lane) & 1ull;
     const uint64_t arg = in ? op->arg[lane] : 0;
-    bool bad = lanes == 0 || kind > GAL_OP_FREE;
+    const bool is_malloc = kind == GAL_OP_MALLOC || kind == GAL_OP_MALLOC_SPREAD;
+    bool bad = lanes == 0 || (!is_malloc && kind != GAL_OP_FREE);
     uint32_t path = 0;
-    if (!bad && in && kind == GAL_OP_MALLOC) {
+    if (!bad && in && is_malloc) {
       bad = freed && arg == 0;
       path = A::request_path(arg);
     } else if (!bad && in) {
       if (freed) {
         const uint64_t src = arg / 64, src_lane = arg % 64;
-        bad = arg >= i * 64 || ops[src].kind != GAL_OP_MALLOC || !((ops[src].lanes >> src_lane) & 1ull);
+        const uint32_t src_kind = arg >= i * 64 ? GAL_OP_FREE : ops[src].kind;
+        bad = (src_kind != GAL_OP_MALLOC && src_kind != GAL_OP_MALLOC_SPREAD) || !((ops[src].lanes >> src_lane) & 1ull);
         if (!bad && (lanes & (lanes - 1)) != 0) bad = A::request_path(ops[src].arg[src_lane]) == 2u;
         if (!bad) bad = ((atom::fetch_or(&freed[src], 1ull << src_lane) >> src_lane) & 1ull) != 0;
       }
       if (!bad) bad = results[arg] == GAL_NULL_OFFSET;
     }
-    if (freed && kind == GAL_OP_MALLOC) {
+    if (freed && is_malloc) {
       const uint64_t large = wave::ballot(in && path != 0u), small = wave::ballot(in && path == 0u);
       bad = bad || (large != 0 && small != 0);
     }
@@ -132,8 +136,9 @@ __global__ void __launch_bounds__(64) group_replay_kernel(A* a, const gal_group_
     }
     if (!in) {
       results[i * 64 + lane] = GAL_NULL_OFFSET;
-    } else if (kind == GAL_OP_MALLOC) {
-      void* p = a->template malloc_at<false>(arg, op->tid[lane], op->slot);
+    } else if (is_malloc) {
+      void* p = kind == GAL_OP_MALLOC_SPREAD ? a->template malloc_at<true>(arg, op->tid[lane], op->slot)
+                                             : a->template malloc_at<false>(arg, op->tid[lane], op->slot);
       results[i * 64 + lane] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
     } else {
       a->free(base + results[arg]);
@@ -608,7 +613,7 @@ int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint
   for (uint64_t i = 0; i < n; i++) {
     const gal_group_op& op = ops[i];
     if (op.lanes == 0) return GAL_ERR_INVALID;
-    if (op.kind == GAL_OP_MALLOC) {
+    if (op.kind == GAL_OP_MALLOC || op.kind == GAL_OP_MALLOC_SPREAD) {
       bool large = false, small = false;
       for (uint64_t m = op.lanes; m; m &= m - 1) {
         const uint64_t size = op.arg[__builtin_ctzll(m)];
@@ -622,7 +627,8 @@ int gal_group_replay(gal_allocator* h, const gal_group_op* ops, uint64_t n, uint
         const uint64_t j = op.arg[__builtin_ctzll(m)];
         if (j >= i * 64 || freed[j]) return GAL_ERR_INVALID;
         const gal_group_op& src = ops[j / 64];
-        if (src.kind != GAL_OP_MALLOC || !((src.lanes >> (j % 64)) & 1ull)) return GAL_ERR_INVALID;
+        if ((src.kind != GAL_OP_MALLOC && src.kind != GAL_OP_MALLOC_SPREAD) || !((src.lanes >> (j % 64)) & 1ull))
+          return GAL_ERR_INVALID;
         if (several && h->ops->path_of(src.arg[j % 64]) == 2) return GAL_ERR_INVALID;
         freed[j] = 1;
       }

@@ -155,11 +155,14 @@ int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* result
  * are active at a call in a reference kernel enter Gallatin::malloc
  * together (gallatin.cuh:1186-1276, teams by class at :881-885, one claim
  * per team at block.cuh:245-281; frees grouped by block at :1774-1827).
- * The claims use the reference's pinned-slot walk from `slot` for every
- * group (the overflow-slot spreading that concurrent claims use is off), so
- * the results are a function of the op stream alone and the CPU oracle
- * reproduces them bit for bit.
- *   GAL_OP_MALLOC: arg[l] = request size of lane l. The lanes' sizes are all
+ * GAL_OP_MALLOC claims use the reference's pinned-slot walk from `slot` for
+ * every group; GAL_OP_MALLOC_SPREAD claims are the ones malloc makes under
+ * load (when more than one lane claims, a group's first pass starts at slot
+ * `slot` % usable of its class, pinned or overflow, and fills it if empty
+ * unless the class is out of memory). Either way the results are a function
+ * of the op stream alone and the CPU oracle reproduces them bit for bit.
+ *   GAL_OP_MALLOC, GAL_OP_MALLOC_SPREAD:
+ *                  arg[l] = request size of lane l. The lanes' sizes are all
  *                  on the slice path (up to min_size * 2048 bytes) or all
  *                  above it (the block and segment requests are served one
  *                  lane at a time in lane order).
@@ -171,8 +174,9 @@ int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* result
  * 0 (free), GAL_NULL_OFFSET (a failed malloc, or a lane not in the op).
  * No counterpart in the reference (whose concurrent results are not
  * reproducible); it pins the group claim semantics against the oracle. */
+#define GAL_OP_MALLOC_SPREAD 2u
 typedef struct gal_group_op {
-  uint32_t kind;    /* GAL_OP_MALLOC or GAL_OP_FREE */
+  uint32_t kind;    /* GAL_OP_MALLOC, GAL_OP_MALLOC_SPREAD or GAL_OP_FREE */
   uint32_t slot;    /* CU id stand-in of the wavefront (the reference's %smid) */
   uint64_t lanes;   /* lanes taking part, nonzero */
   uint64_t arg[64]; /* per lane, see above */

@@ -244,6 +244,17 @@ struct Veb {
     for (auto& l : layers) std::fill(l.bits.begin(), l.bits.end(), 0ull);
   }
 
+  // The GPU's out-of-memory shortcut reads: the top layer's first word is
+  // empty and it is the only word of its layer or summarises a lower layer
+  // (veb.hpp top_looks_empty); the number of items (the free-segment gauge
+  // for the segment tree).
+  bool top_looks_empty() const { return layers[0].bits[0] == 0ull && (num_layers > 1 || layers[0].num_blocks == 1); }
+  uint64_t count_items() const {
+    uint64_t n = 0;
+    for (uint64_t w : layers[num_layers - 1].bits) n += (uint64_t)popc(w);
+    return n;
+  }
+
   // veb.cuh:697-712
   static bool float_up(int& layer, uint64_t& high, int& low) {
     layer -= 1;
@@ -518,7 +529,8 @@ struct Alloc {
   Veb segment_tree;
   std::vector<Veb> sub_trees;
   Table table;
-  std::vector<std::vector<uint32_t>> slots;  // pinned blocks: block id + 1
+  std::vector<std::vector<uint32_t>> slots;     // pinned blocks: block id + 1
+  std::vector<std::vector<uint32_t>> overflow;  // overflow slots of each class (block id + 1)
   uint64_t faults = 0;
 
   Alloc(uint64_t seg, uint64_t min, uint64_t max, uint64_t nseg, uint64_t sd)
@@ -629,9 +641,24 @@ struct Alloc {
     }
   }
 
+  // slot `index` of class `tree`: its pinned slots first, then its overflow slots
+  uint32_t& slot_ref(uint32_t tree, uint64_t index) {
+    const uint64_t pinned = slots[tree].size();
+    return index < pinned ? slots[tree][index] : overflow[tree][index - pinned];
+  }
+
+  // Slots of class `tree` a group may start at under load: its pinned slots
+  // and overflow slots holding at most 1/16 of the pool in blocks, at least
+  // its pinned slots (the GPU's usable_slots_of, kSlotShare = 16).
+  uint64_t usable_slots(uint32_t tree) const {
+    const uint64_t budget = table.num_segments * SEG / 16 / (table.tree_alloc_size(tree) * 4096);
+    const uint64_t lo = slots[tree].size(), hi = lo + overflow[tree].size();
+    return budget < lo ? lo : budget > hi ? hi : budget;
+  }
+
   // gallatin.cuh:1083-1138 replace_block (+ Q6: no trap on a racing refill)
   bool replace_block(uint32_t tree, int slot_index, uint64_t block, uint64_t tid) {
-    uint32_t& s = slots[tree][slot_index];
+    uint32_t& s = slot_ref(tree, (uint64_t)slot_index);
     if (s != (uint32_t)(block + 1)) return true;  // swap_out_block CAS failed
     s = 0u;
     int64_t nb = request_new_block_from_tree(tree, tid);
@@ -826,11 +853,60 @@ struct Alloc {
     uint32_t c0 = 0;
     bool empty = false, replace = false, completed = false, stale = false;
     uint32_t give_back = 0;
+    // the under-load first look: the class found out of memory; the team
+    // refills the empty slot the look landed on (a failed refill then sends
+    // it to the pinned walk in the next pass instead of failing)
+    bool dry = false, look_refill = false;
   };
 
-  // one pass of the slice claim over the lanes of `looking`
+  // The GPU's first look under load (more than one lane of the wave
+  // claiming, the first pass of a malloc call): the team starts at slot
+  // hint % usable of its class, pinned or overflow, instead of the pinned
+  // walk. A slot holding a block is taken. An empty slot is taken (and
+  // refilled) unless the class is out of memory: its tree empty with no free
+  // segment left, or no free segment left and the leader's hashed search
+  // finding nothing; when it is, and every pinned slot holds 0 as well, the
+  // team fails at once. A class whose usable slots are its pinned slots
+  // looks at slot hint % pinned, where the walk starts, and only the failure
+  // test applies to an empty slot there. Returns whether the look placed
+  // the team (t.index set).
+  bool first_look(Team& t, uint32_t hint, uint64_t leader_tid) {
+    const uint64_t pinned = slots[t.tree].size();
+    const uint64_t usable = usable_slots(t.tree);
+    const bool spreads = usable > pinned;
+    const uint64_t si = hint % (spreads ? usable : pinned);
+    const uint64_t free_segments = segment_tree.count_items();
+    const bool tree_dry = sub_trees[t.tree].top_looks_empty() && free_segments == 0;
+    uint64_t empty_pinned = 0;
+    for (uint32_t e : slots[t.tree]) empty_pinned += e == 0u;
+    const bool all_empty = empty_pinned >= pinned;
+    const uint32_t se = slot_ref(t.tree, si);
+    if (se != 0u) {
+      t.index = (int)si;
+      return true;
+    }
+    if (spreads) {
+      const bool spare = free_segments != 0;
+      t.dry = tree_dry || (!spare && sub_trees[t.tree].find_random_valid_index(leader_tid) == FAIL);
+      if (!t.dry || all_empty) {
+        t.index = (int)si;
+        t.look_refill = !t.dry;
+        return true;
+      }
+      return false;
+    }
+    if (tree_dry && all_empty) {
+      t.dry = true;
+      t.index = (int)si;
+      return true;
+    }
+    return false;
+  }
+
+  // one pass of the slice claim over the lanes of `looking`; `look`: the
+  // teams take the first look under load (see first_look)
   void team_pass(uint64_t looking, const uint32_t* tree, const uint32_t* count, const uint64_t* tid, uint32_t slot,
-                 uint64_t* result, bool* done, bool* oom) {
+                 bool look, uint64_t* result, bool* done, bool* oom) {
     std::vector<Team> teams;
     for (uint64_t todo = looking; todo;) {
       const int first = ffs64(todo) - 1;
@@ -845,8 +921,8 @@ struct Alloc {
     }
     // claims: each team on its own class's slot and block
     for (Team& t : teams) {
-      t.index = valid_block_index(t.tree, slot);
-      t.entry = slots[t.tree][t.index];
+      if (!(look && first_look(t, slot, tid[ffs64(t.lanes) - 1]))) t.index = valid_block_index(t.tree, slot);
+      t.entry = slot_ref(t.tree, (uint64_t)t.index);
       t.empty = t.entry == 0u;
       if (t.empty) continue;
       t.block = t.entry - 1;
@@ -886,9 +962,14 @@ struct Alloc {
     for (Team& t : teams) {
       const uint64_t leader_tid = tid[ffs64(t.lanes) - 1];
       if (t.empty) {
-        int64_t fresh = request_new_block_from_tree(t.tree, leader_tid);
-        if (fresh >= 0 && slots[t.tree][t.index] == 0u) {
-          slots[t.tree][t.index] = (uint32_t)(fresh + 1);
+        // a class the first look found out of memory takes no block request
+        int64_t fresh = t.dry ? -1 : request_new_block_from_tree(t.tree, leader_tid);
+        uint32_t& s = slot_ref(t.tree, (uint64_t)t.index);
+        if (fresh >= 0 && s == 0u) {
+          s = (uint32_t)(fresh + 1);
+        } else if (fresh < 0 && t.look_refill) {
+          // the look's slot could not be refilled: the next pass walks the
+          // pinned slots, where blocks can still be
         } else {
           if (fresh >= 0) free_block((uint64_t)fresh);
           for (uint64_t m = t.lanes; m; m &= m - 1) {
@@ -921,7 +1002,10 @@ struct Alloc {
   // range are served one lane at a time in lane order (they take no team
   // claim); a call site mixing both kinds is not modelled (the order of the
   // two branches is up to the compiler).
-  int group_malloc(uint64_t lanes, const uint64_t* size, const uint64_t* tid, uint32_t slot, uint64_t* out) {
+  // `spread`: the GPU's malloc under load (first look, overflow slots);
+  // false: the reference's pinned walk for every pass.
+  int group_malloc(uint64_t lanes, const uint64_t* size, const uint64_t* tid, uint32_t slot, bool spread,
+                   uint64_t* out) {
     uint32_t tree[64] = {}, count[64] = {};
     bool large = false, small = false;
     for (uint64_t m = lanes; m; m &= m - 1) {
@@ -968,8 +1052,10 @@ struct Alloc {
       for (int l = 0; l < 64; l++) done[l] = false;
       for (uint64_t m = calling; m; m &= m - 1) calls[ffs64(m) - 1]++;
       uint64_t looking = calling;
-      while (looking) {
-        team_pass(looking, tree, count, tid, slot, result, done, oom);
+      for (bool first = true; looking; first = false) {
+        // the look needs more than one lane claiming (of any class)
+        team_pass(looking, tree, count, tid, slot, spread && first && (looking & (looking - 1)) != 0, result, done,
+                  oom);
         uint64_t still = 0;
         for (uint64_t m = looking; m; m &= m - 1) {
           const int l = ffs64(m) - 1;
@@ -1051,6 +1137,7 @@ struct Alloc {
     uint64_t n = 128;
     for (uint32_t i = 0; i < num_trees; i++) {
       slots.emplace_back(n, 0u);
+      overflow.emplace_back(overflow_slots(n), 0u);
       n = n / 2;
       if (n < 4) n = 4;
     }
@@ -1065,7 +1152,7 @@ struct Alloc {
   // The GPU gives a class with fewer than 64 pinned slots overflow slots up
   // to 64, used only when more than one lane of a wave claims at once; with
   // one lane they stay empty. The snapshot lists each class's pinned slots
-  // followed by its (empty) overflow slots.
+  // followed by its overflow slots.
   static uint64_t overflow_slots(uint64_t pinned) { return pinned < 64 ? 64 - pinned : 0; }
 
   uint64_t total_slots() const {
@@ -1113,10 +1200,9 @@ struct Alloc {
       put(&table.q_free[s], 4);
       put(&table.q_final[s], 4);
     }
-    for (auto& s : slots) {
-      put(s.data(), s.size() * 4);
-      const std::vector<uint32_t> overflow(overflow_slots(s.size()), 0u);
-      if (!overflow.empty()) put(overflow.data(), overflow.size() * 4);
+    for (size_t c = 0; c < slots.size(); c++) {
+      put(slots[c].data(), slots[c].size() * 4);
+      if (!overflow[c].empty()) put(overflow[c].data(), overflow[c].size() * 4);
     }
     size_t used = (size_t)(p - out);
     size_t total = snapshot_bytes();
@@ -1264,8 +1350,8 @@ int orc_group_replay(orc_alloc* h, const gal_group_op* ops, uint64_t n, uint64_t
     const gal_group_op& op = ops[i];
     if (op.lanes == 0) return GAL_ERR_INVALID;
     uint64_t* out = results + i * 64;
-    if (op.kind == GAL_OP_MALLOC) {
-      int rc = h->a.group_malloc(op.lanes, op.arg, op.tid, op.slot, out);
+    if (op.kind == GAL_OP_MALLOC || op.kind == GAL_OP_MALLOC_SPREAD) {
+      int rc = h->a.group_malloc(op.lanes, op.arg, op.tid, op.slot, op.kind == GAL_OP_MALLOC_SPREAD, out);
       if (rc != GAL_OK) return rc;
       continue;
     }
@@ -1274,8 +1360,9 @@ int orc_group_replay(orc_alloc* h, const gal_group_op* ops, uint64_t n, uint64_t
     for (uint64_t m = op.lanes; m; m &= m - 1) {
       const int l = __builtin_ctzll(m);
       const uint64_t j = op.arg[l];
-      if (j >= i * 64 || !((ops[j / 64].lanes >> (j % 64)) & 1ull) || ops[j / 64].kind != GAL_OP_MALLOC ||
-          results[j] == GAL_NULL_OFFSET)
+      const uint32_t src_kind = ops[j / 64].kind;
+      if (j >= i * 64 || !((ops[j / 64].lanes >> (j % 64)) & 1ull) ||
+          (src_kind != GAL_OP_MALLOC && src_kind != GAL_OP_MALLOC_SPREAD) || results[j] == GAL_NULL_OFFSET)
         return GAL_ERR_INVALID;
       offsets[l] = results[j];
     }
@@ -1288,9 +1375,9 @@ int orc_group_replay(orc_alloc* h, const gal_group_op* ops, uint64_t n, uint64_t
 }
 
 int orc_group_malloc(orc_alloc* h, uint64_t lanes, const uint64_t* sizes, const uint64_t* tids, uint32_t slot,
-                     uint64_t* out) {
+                     int spread, uint64_t* out) {
   if (!h || !lanes || !sizes || !tids || !out) return GAL_ERR_INVALID;
-  return h->a.group_malloc(lanes, sizes, tids, slot, out);
+  return h->a.group_malloc(lanes, sizes, tids, slot, spread != 0, out);
 }
 
 int orc_group_free(orc_alloc* h, uint64_t lanes, const uint64_t* byte_offsets) {

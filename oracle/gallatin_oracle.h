@@ -65,9 +65,10 @@ int orc_replay(orc_alloc* a, const gal_op* ops, uint64_t n, uint64_t* results);
  * one claim per team, frees grouped by block); results[i * 64 + l]. */
 int orc_group_replay(orc_alloc* a, const gal_group_op* ops, uint64_t n, uint64_t* results);
 /* One such call site at a time: malloc by the lanes of `lanes` (sizes,
- * tids and out indexed by lane), or free of the byte offsets of `lanes`. */
+ * tids and out indexed by lane; spread != 0: the claims malloc makes under
+ * load, GAL_OP_MALLOC_SPREAD), or free of the byte offsets of `lanes`. */
 int orc_group_malloc(orc_alloc* a, uint64_t lanes, const uint64_t* sizes, const uint64_t* tids, uint32_t slot,
-                     uint64_t* out);
+                     int spread, uint64_t* out);
 int orc_group_free(orc_alloc* a, uint64_t lanes, const uint64_t* byte_offsets);
 int orc_snapshot(orc_alloc* a, void* buf, uint64_t cap, uint64_t* needed);
 /* count of internal consistency faults seen (reference traps) */

@@ -56,7 +56,7 @@ _SIG = {
     "orc_free": (ctypes.c_int, [vp, u64]),
     "orc_replay": (ctypes.c_int, [vp, ctypes.POINTER(_Op), u64, pu64]),
     "orc_group_replay": (ctypes.c_int, [vp, ctypes.POINTER(_GroupOp), u64, pu64]),
-    "orc_group_malloc": (ctypes.c_int, [vp, u64, pu64, pu64, u32, pu64]),
+    "orc_group_malloc": (ctypes.c_int, [vp, u64, pu64, pu64, u32, ctypes.c_int, pu64]),
     "orc_group_free": (ctypes.c_int, [vp, u64, pu64]),
     "orc_snapshot": (ctypes.c_int, [vp, vp, u64, pu64]),
     "orc_faults": (u64, [vp]),
@@ -209,12 +209,13 @@ class Oracle:
             raise RuntimeError(f"orc_group_replay failed: {rc}")
         return np.frombuffer(out, dtype=np.uint64, count=64 * len(ops)).copy().reshape(len(ops), 64)
 
-    def group_malloc(self, lanes, sizes, tids, slot):
-        """malloc by the lanes of `lanes` together; sizes/tids: 64 entries by lane."""
+    def group_malloc(self, lanes, sizes, tids, slot, spread=False):
+        """malloc by the lanes of `lanes` together; sizes/tids: 64 entries by
+        lane. spread: the claims malloc makes under load (GAL_OP_MALLOC_SPREAD)."""
         sz = (u64 * 64)(*[int(x) for x in sizes])
         td = (u64 * 64)(*[int(x) for x in tids])
         out = (u64 * 64)()
-        rc = lib.orc_group_malloc(self.h, lanes, sz, td, slot, out)
+        rc = lib.orc_group_malloc(self.h, lanes, sz, td, slot, int(bool(spread)), out)
         if rc != 0:
             raise RuntimeError(f"orc_group_malloc failed: {rc}")
         return list(out)

@@ -32,9 +32,11 @@ def lane_mask(rng):
 
 
 def build(o, seed, n_ops, slice_sizes, large_sizes=(), free_p=0.4, large_p=0.05, slots=256, seg=16 << 20,
-          mn=16, mx=4096):
+          mn=16, mx=4096, spread_p=0.0):
     """Generate n_ops ops, running each on the oracle `o` as it is made.
-    Returns (ops, results) with results an (n_ops, 64) list of offsets."""
+    Returns (ops, results) with results an (n_ops, 64) list of offsets.
+    A malloc op of slice sizes is GAL_OP_MALLOC_SPREAD (kind 2, the claims
+    malloc makes under load) with probability spread_p."""
     rng = random.Random(seed)
     large_sizes = [x for x in large_sizes if oracle.route(seg, mn, mx, x)[0] != 0]
     ops, results = [], []
@@ -80,9 +82,10 @@ def build(o, seed, n_ops, slice_sizes, large_sizes=(), free_p=0.4, large_p=0.05,
             sizes = [rng.choice(slice_sizes)] * 64
         else:
             sizes = [rng.choice(slice_sizes) for _ in range(64)]
-        out = o.group_malloc(lanes, sizes, tids, slot)
+        spread = oracle.route(seg, mn, mx, sizes[(lanes & -lanes).bit_length() - 1])[0] == 0 and rng.random() < spread_p
+        out = o.group_malloc(lanes, sizes, tids, slot, spread=spread)
         idx = len(ops)
-        ops.append((0, slot, lanes, sizes, tids))
+        ops.append((2 if spread else 0, slot, lanes, sizes, tids))
         row = [NULL] * 64
         for lane in range(64):
             if (lanes >> lane) & 1:

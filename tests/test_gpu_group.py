@@ -4,7 +4,11 @@ gal_group_replay runs each op with exactly its lanes active, so the team
 claims (one add per class team, lane-order prefixes, write-off of straddled
 slices, replacement on slice 4095) and the per-block free teams are
 deterministic; the results and the whole metadata image must equal the
-oracle's bit for bit.
+oracle's bit for bit. The spread cases mix in GAL_OP_MALLOC_SPREAD ops, the
+claims malloc makes under load: a group's first pass starts at slot
+hint % usable of its class (pinned or overflow), fills an empty one unless
+the class is out of memory, and fails at once when it is and every pinned
+slot holds 0.
 """
 import sys
 
@@ -28,23 +32,28 @@ def say(msg):
 
 
 CASES = [
-    # name, pool, (seg, min, max), slice sizes, large sizes, ops
-    ("slices", 1 << 30, (16 * MiB, 16, 4096), [16, 32, 48, 64, 256, 1000, 2048, 4096], (), 1500),
-    ("one_class_full_waves", 1 << 30, (16 * MiB, 16, 4096), [16], (), 800),
-    ("big_slices", 1 << 30, (16 * MiB, 16, 4096), [16, 4096, 8192, 12000, 16384, 32768], (), 1200),
-    ("large", 4 << 30, (16 * MiB, 16, 4096), [16, 256, 4096, 8192], (40000, MiB, 16 * MiB, 40 * MiB), 800),
-    ("exhaustion", 64 * MiB, (MiB, 16, 256), [16, 32, 256, 1024, 4096], (65536, 2 * MiB), 1500),
-    ("min128", 1 << 30, (4 * MiB, 128, 1024), [16, 100, 128, 512, 1024, 3000], (200000,), 1000),
+    # name, pool, (seg, min, max), slice sizes, large sizes, ops, share of spread malloc ops
+    ("slices", 1 << 30, (16 * MiB, 16, 4096), [16, 32, 48, 64, 256, 1000, 2048, 4096], (), 1500, 0.0),
+    ("one_class_full_waves", 1 << 30, (16 * MiB, 16, 4096), [16], (), 800, 0.0),
+    ("big_slices", 1 << 30, (16 * MiB, 16, 4096), [16, 4096, 8192, 12000, 16384, 32768], (), 1200, 0.0),
+    ("large", 4 << 30, (16 * MiB, 16, 4096), [16, 256, 4096, 8192], (40000, MiB, 16 * MiB, 40 * MiB), 800, 0.0),
+    ("exhaustion", 64 * MiB, (MiB, 16, 256), [16, 32, 256, 1024, 4096], (65536, 2 * MiB), 1500, 0.0),
+    ("min128", 1 << 30, (4 * MiB, 128, 1024), [16, 100, 128, 512, 1024, 3000], (200000,), 1000, 0.0),
+    ("spread_slices", 1 << 30, (16 * MiB, 16, 4096), [16, 32, 48, 64, 256, 1000, 2048, 4096], (), 1500, 0.7),
+    ("spread_big_slices", 2 << 30, (16 * MiB, 16, 4096), [64, 512, 4096, 8192, 16384, 32768], (), 1200, 0.7),
+    ("spread_large", 4 << 30, (16 * MiB, 16, 4096), [16, 256, 4096], (40000, MiB, 40 * MiB), 800, 0.5),
+    ("spread_exhaustion", 64 * MiB, (MiB, 16, 256), [16, 32, 256, 1024, 4096], (65536, 2 * MiB), 1500, 0.7),
+    ("spread_small_pool_oom", 256 * MiB, (16 * MiB, 16, 4096), [1024, 2048, 4096], (), 1500, 0.8),
 ]
 
 
-@pytest.mark.parametrize("name,pool,triple,sizes,large,n", CASES, ids=[c[0] for c in CASES])
-def test_group_replay_bit_exact(name, pool, triple, sizes, large, n):
+@pytest.mark.parametrize("name,pool,triple,sizes,large,n,spread_p", CASES, ids=[c[0] for c in CASES])
+def test_group_replay_bit_exact(name, pool, triple, sizes, large, n, spread_p):
     seg, mn, mx = triple
     seed = 31
     o = oracle.Oracle(pool, seed, segment=seg, min_size=mn, max_size=mx)
     ops, results, live = group_streams.build(o, seed, n, sizes, large, seg=seg, mn=mn, mx=mx,
-                                             large_p=0.08 if large else 0.0)
+                                             large_p=0.08 if large else 0.0, spread_p=spread_p)
     expect = np.array(results, dtype=np.uint64)
     lanes = sum(bin(op[2]).count("1") for op in ops)
     say(f"[group] {name}: {len(ops)} ops, {lanes} lane calls, {len(live)} live at the end")
