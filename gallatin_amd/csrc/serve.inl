@@ -9,7 +9,7 @@
 //   waiting  -> admitted once the request has arrived (device wall clock)
 //               and the pool has room for all of its pages
 //   growing  -> one page per loop iteration: malloc, write the page header
-//               (link to the previous page + content words), read the
+//               (link to the previous page + a content word), read the
 //               previous page's content back into the checksum
 //   draining -> one page per loop iteration: read the content back, free
 //   (finished: checksum and times written, back to idle)
@@ -116,26 +116,20 @@ __device__ inline uint64_t wave_sum64(uint64_t x) {
   return x;
 }
 
-// Page headers are read and written 16 bytes at a time (pages are aligned
-// to at least 16 bytes): word 0 is the link, words 1 .. w-1 the content, w
-// in {2, 4, 8}.
+// A page header (word 0 the link, word 1 the content) is read and written as
+// one 16-byte access (pages are aligned to at least 16 bytes). Reading back
+// 64 bytes of content per page instead (four 16-byte accesses) cost the
+// short-sequence loop 12 % of its rate and the long one 21 %: its
+// iterations wait on memory accesses in flight, not on bytes.
 typedef uint64_t page_pair __attribute__((ext_vector_type(2)));
 
-__device__ inline void write_page(uint64_t* p, uint64_t link, uint64_t id, uint32_t page, uint32_t w) {
-  page_pair* q = (page_pair*)p;
-  q[0] = page_pair{link, gw::page_word(id, page, 1)};
-  if (w > 2) q[1] = page_pair{gw::page_word(id, page, 2), gw::page_word(id, page, 3)};
-  if (w > 4) {
-    q[2] = page_pair{gw::page_word(id, page, 4), gw::page_word(id, page, 5)};
-    q[3] = page_pair{gw::page_word(id, page, 6), gw::page_word(id, page, 7)};
-  }
+__device__ inline void write_page(uint64_t* p, uint64_t link, uint64_t id, uint32_t page) {
+  *(page_pair*)p = page_pair{link, gw::page_word(id, page)};
 }
 
-// fold words 1 .. w-1 of a header already loaded as four pairs; returns the link
-__device__ inline uint64_t fold_page(page_pair a, page_pair b, page_pair c, page_pair d, uint32_t w, uint64_t& h) {
+// fold the content word of a header already loaded; returns the link
+__device__ inline uint64_t fold_page(page_pair a, uint64_t& h) {
   h = gw::fold(h, a.y);
-  if (w > 2) h = gw::fold(gw::fold(h, b.x), b.y);
-  if (w > 4) h = gw::fold(gw::fold(gw::fold(gw::fold(h, c.x), c.y), d.x), d.y);
   return a.x;
 }
 
@@ -209,17 +203,8 @@ __global__ void __launch_bounds__(256)
     // admit, the admission budget.
     uint64_t* const top_page = page;
     const bool reading = (state == kSeqGrowing || state == kSeqDraining) && page != nullptr;
-    const uint32_t top_words = gw::page_words(r.page_bytes);
-    page_pair q0 = {0, 0}, q1 = {0, 0}, q2 = {0, 0}, q3 = {0, 0};
-    if (reading) {
-      const page_pair* q = (const page_pair*)page;
-      q0 = q[0];
-      if (top_words > 2) q1 = q[1];
-      if (top_words > 4) {
-        q2 = q[2];
-        q3 = q[3];
-      }
-    }
+    page_pair q0 = {0, 0};
+    if (reading) q0 = *(const page_pair*)page;
     // idle lanes that take tickets in this iteration
     uint64_t idle = wave::ballot(state == kSeqIdle);
     if (idle && wave::popc(idle) < kTicketBatch && idle_age < kTicketWait &&
@@ -360,10 +345,9 @@ __global__ void __launch_bounds__(256)
     if (grow && claimed > step) {
       uint64_t* p = (uint64_t*)reserved;
       reserved += stride;
-      const uint32_t w = gw::page_words(r.page_bytes);
-      write_page(p, (uint64_t)page | (mark ? kRunStart : 0ull), r.id, step, w);
+      write_page(p, (uint64_t)page | (mark ? kRunStart : 0ull), r.id, step);
       mark = false;
-      if (page) (void)fold_page(q0, q1, q2, q3, w, h);
+      if (page) (void)fold_page(q0, h);
       page = p;
       step++;
       if (in_window) nw++;
@@ -382,7 +366,7 @@ __global__ void __launch_bounds__(256)
     if (draining) {
       if (drain) {
         first = page;
-        const uint64_t link = fold_page(q0, q1, q2, q3, top_words, h);
+        const uint64_t link = fold_page(q0, h);
         if (in_window) nw++;
         page = (uint64_t*)(link & ~kRunStart);
         drained++;

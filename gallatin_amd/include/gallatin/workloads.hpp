@@ -33,14 +33,14 @@ __host__ __device__ inline uint64_t payload_tag(uint64_t seed, uint64_t index) {
 }
 
 // ---- serving: the header a sequence writes into each of its pages.
-// Word 0 links to the sequence's previous page; words 1 .. page_words-1
-// hold content derived from (request id, page index). A sequence folds every
-// content word it reads back into its checksum, so the checksum is a pure
-// function of (id, pages, page bytes) unless another sequence wrote into one
-// of its pages.
-__host__ __device__ inline uint32_t page_words(uint32_t page_bytes) { return page_bytes >= 64 ? 8u : page_bytes / 8u; }
-__host__ __device__ inline uint64_t page_word(uint64_t id, uint64_t page, uint64_t k) {
-  return utils::splitmix64(utils::splitmix64(id) ^ ((page << 3) | k));
+// A page's 16-byte header (pages are at least 16 bytes): word 0 links to the
+// sequence's previous page, word 1 holds content derived from (request id,
+// page index). A sequence folds every content word it reads back into its
+// checksum, so the checksum is a pure function of (id, pages) unless another
+// sequence wrote into one of its pages. One word of payload written and read
+// back per page, like the one tag per allocation of the other workloads.
+__host__ __device__ inline uint64_t page_word(uint64_t id, uint64_t page) {
+  return utils::splitmix64(utils::splitmix64(id) ^ ((page << 3) | 1));
 }
 __host__ __device__ inline uint64_t fold(uint64_t h, uint64_t x) { return utils::splitmix64(h ^ x); }
 

@@ -1,14 +1,15 @@
 """Expected results of the serving loop, computed on the host with numpy.
 
-A sequence of `pages` pages of `page_bytes` bytes, request id `id`:
- * page s holds content words page_word(id, s, k) for k = 1 .. w-1, with
-   w = min(page_bytes / 8, 8) (word 0 links to the previous page);
- * growing: after page s >= 1 is allocated, the words of page s-1 are read
+A sequence of `pages` pages (of any size: only the 16-byte header is
+written), request id `id`:
+ * page s holds the content word page_word(id, s) (word 0 links to the
+   previous page);
+ * growing: after page s >= 1 is allocated, the word of page s-1 is read
    back and folded into the checksum h (h starts at id);
  * draining: pages L-1 down to 0 are read back and folded, then freed;
  * the result is fold(h, pages).
-fold(h, x) = splitmix64(h ^ x); page_word(id, s, k) = splitmix64(splitmix64(id) ^ (s << 3 | k)).
-Vectorised over requests: the loop runs over page index and word.
+fold(h, x) = splitmix64(h ^ x); page_word(id, s) = splitmix64(splitmix64(id) ^ (s << 3 | 1)).
+Vectorised over requests: the loop runs over page index.
 """
 import numpy as np
 
@@ -38,19 +39,15 @@ def splitmix64_int(x):
 def expected_checksums(ids, pages, page_bytes):
     ids = np.asarray(ids, dtype=np.uint64)
     pages = np.asarray(pages, dtype=np.int64)
-    page_bytes = np.asarray(page_bytes, dtype=np.int64)
-    words = np.where(page_bytes >= 64, 8, page_bytes // 8)
+    assert np.all(np.asarray(page_bytes) >= 16)
     base = splitmix64(ids)
     h = ids.copy()
     top = int(pages.max()) if len(pages) else 0
 
     def fold_page(h, s, active):
         s64 = np.asarray(s, dtype=np.uint64)
-        for k in range(1, 8):
-            use = active & (k < words)
-            w = splitmix64(base ^ ((s64 << np.uint64(3)) | np.uint64(k)))
-            h = np.where(use, splitmix64(h ^ w), h)
-        return h
+        w = splitmix64(base ^ ((s64 << np.uint64(3)) | np.uint64(1)))
+        return np.where(active, splitmix64(h ^ w), h)
 
     # growth: page s-1 read back after page s is allocated, s = 1 .. pages-1
     for s in range(1, top):
@@ -63,15 +60,13 @@ def expected_checksums(ids, pages, page_bytes):
 
 def expected_checksum_scalar(rid, pages, page_bytes):
     """The same on Python ints, one request (small cases only)."""
-    w = 8 if page_bytes >= 64 else page_bytes // 8
-    word = lambda s, k: splitmix64_int(splitmix64_int(rid) ^ ((s << 3) | k))  # noqa: E731
+    assert page_bytes >= 16
+    word = lambda s: splitmix64_int(splitmix64_int(rid) ^ ((s << 3) | 1))  # noqa: E731
     h = rid
     for s in range(1, pages):
-        for k in range(1, w):
-            h = splitmix64_int(h ^ word(s - 1, k))
+        h = splitmix64_int(h ^ word(s - 1))
     for s in range(pages - 1, -1, -1):
-        for k in range(1, w):
-            h = splitmix64_int(h ^ word(s, k))
+        h = splitmix64_int(h ^ word(s))
     return splitmix64_int(h ^ pages)
 
 

@@ -107,11 +107,28 @@ def test_checksum_vectorised_matches_scalar():
 
 
 def test_checksum_depends_on_every_read():
-    # a plausible slip (a dropped page, another id, another size) changes it
+    # a plausible slip (a dropped page, another id) changes it; the page size
+    # does not enter (only the 16-byte header is written)
     base = serving_ref.expected_checksum_scalar(11, 6, 64)
     assert base != serving_ref.expected_checksum_scalar(11, 5, 64)
     assert base != serving_ref.expected_checksum_scalar(12, 6, 64)
-    assert base != serving_ref.expected_checksum_scalar(11, 6, 32)
+    assert base == serving_ref.expected_checksum_scalar(11, 6, 16) == serving_ref.expected_checksum_scalar(11, 6, 16384)
+    # skipping any one of the 2 * pages - 1 reads, or reading one page twice, changes it
+    m = (1 << 64) - 1
+    word = lambda s: serving_ref.splitmix64_int(serving_ref.splitmix64_int(11) ^ ((s << 3) | 1))  # noqa: E731
+    reads = list(range(0, 5)) + list(range(5, -1, -1))  # growth reads pages 0..4, drain 5..0
+    assert len(reads) == 2 * 6 - 1
+
+    def checksum(seq):
+        h = 11
+        for s in seq:
+            h = serving_ref.splitmix64_int(h ^ word(s)) & m
+        return serving_ref.splitmix64_int(h ^ 6)
+
+    assert checksum(reads) == base
+    for j in range(len(reads)):
+        assert checksum(reads[:j] + reads[j + 1:]) != base
+        assert checksum(reads[:j] + [reads[j]] + reads[j:]) != base
     assert serving_ref.splitmix64_int(0) == 0xE220A8397B1DCDAF  # splitmix64 known answer (Vigna)
 
 
