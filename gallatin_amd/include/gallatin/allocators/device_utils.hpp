@@ -129,11 +129,24 @@ __device__ inline T cas(T* p, same_t<T> expected, same_t<T> desired) {
                                        __HIP_MEMORY_SCOPE_AGENT);
   return expected;
 }
-// agent-scope fences. fence() is the reference's __threadfence(); the
-// one-sided forms cost half as much (write-back only / invalidate only).
+// Agent-scope fences (fence() is the reference's __threadfence()). On
+// MI355X the release half writes back the calling XCD's whole L2 and the
+// acquire half invalidates it: they are for ordinary (non-atomic) data, i.e.
+// the payload a free hands back. Metadata is ordered with order() below.
 __device__ inline void fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent"); }
 __device__ inline void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ inline void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Orders the calling lane's metadata accesses, which are all agent-scope
+// atomics, loads and stores (performed coherently across the XCDs): waits
+// until every earlier vector memory access has completed, with none of the
+// L2 write-back or invalidation that an agent-scope fence adds for ordinary
+// (non-atomic) data. The signal fences keep the compiler from moving memory
+// accesses across the wait.
+__device__ inline void order() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0), expcnt and lgkmcnt not waited for
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 }  // namespace atom
 

@@ -171,9 +171,9 @@ struct Gallatin {
       return -1;
     }
     count_stat(control, kStatAttach);
-    atom::fence();
+    atom::order();
     sub_trees[tree].insert_force_update(segment);
-    atom::fence();
+    atom::order();
     return (int64_t)segment;
   }
 
@@ -186,7 +186,7 @@ struct Gallatin {
       gather_new_segment(tree);
       release_tree_lock(tree);
     }
-    atom::fence();
+    atom::order();
   }
 
   // find a block for class `tree`, attaching a segment when the class has
@@ -234,7 +234,7 @@ struct Gallatin {
           continue;
         }
       }
-      atom::fence_acquire();
+      atom::order();
       bool was_last = false;
       int64_t block = table.get_block(segment, tree, was_last, control);
       if (was_last) {
@@ -255,23 +255,23 @@ struct Gallatin {
   __device__ __noinline__ void return_block(uint64_t block, uint64_t segment, uint32_t tree) {
     // everything this wave did to the block (its counter reset above all) is
     // performed before the block becomes grantable again
-    atom::fence_release();
+    atom::order();
     const uint64_t nb = table_type::blocks_in_segment(tree);
     const uint32_t slot = table.reserve_segment_slot(block, segment, nb, control);
     // 1.0*slot/nb >= 0.1 && 1.0*(slot-1)/nb < 0.1 for power-of-two nb
     // (reference :1646): re-list the segment once a tenth of it is back
     if (slot >= 1 && 10ull * slot > nb && 10ull * (slot - 1) < nb) {
       sub_trees[tree].insert_force_update(segment);
-      atom::fence();
+      atom::order();
     }
     const bool release = table.finish_freeing_block(segment, nb, control);
-    atom::fence();
+    atom::order();
     if (release) {
       count_stat(control, kStatRelease);
       sub_trees[tree].remove(segment);
-      atom::fence();
+      atom::order();
       if (!table.reset_tree_id(segment, tree)) count_error(kErrChunkReset);
-      atom::fence();
+      atom::order();
       segment_tree.insert_force_update(segment);
       atom::add(ctl_word(control, kCtlFreeSegs), 1u);
     }
@@ -321,10 +321,10 @@ struct Gallatin {
     }
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatReplace);
-    atom::fence();
+    atom::order();
     bool last = false;
     const int64_t fresh = request_new_block_from_tree(tree, tid, last);
-    atom::fence_release();  // the block's counter reset precedes its publication
+    atom::order();  // the block's counter reset precedes its publication
     atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
     if (fresh < 0 && index < pinned_slots_of(tree)) atom::add(empty_pinned_word(tree), 1);
     count_stat(control, kStatReplaceTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
@@ -346,7 +346,7 @@ struct Gallatin {
     count_stat(control, kStatRefill);
     bool last = false;
     const int64_t fresh = request_new_block_from_tree(tree, tid, last);
-    atom::fence_release();
+    atom::order();
     atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
     if (fresh < 0 && pinned) atom::add(empty_pinned_word(tree), 1);
     count_stat(control, kStatRefillTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
@@ -708,7 +708,7 @@ struct Gallatin {
       }
     }
     if (!wave::readlane(locked, leader)) return utils::kFail;
-    atom::fence_acquire();
+    atom::order();
     const uint64_t start = segment_tree.gather_plan(n, lanes);
     uint64_t first = utils::kFail;
     if ((int)wave::lane_id() == leader && start != utils::kFail) {
@@ -719,7 +719,7 @@ struct Gallatin {
       }
     }
     first = wave::readlane64(first, leader);
-    atom::fence_release();
+    atom::order();
     if ((int)wave::lane_id() == leader) release_tree_lock(kNumTrees);
     return first == utils::kFail ? utils::kFail : first * kBlocksPerSegment * kSlicesPerBlock;
   }

@@ -147,7 +147,7 @@ struct alloc_table {
     atom::exch(&s->queue_free, 0u);
     atom::exch(&s->queue_final, 0u);
     // grants open only after the queue state above is in place
-    atom::fence_release();
+    atom::order();
     atom::exch(&s->active, (int32_t)nb - 1);
     return true;
   }
@@ -233,7 +233,7 @@ struct alloc_table {
   // reference memory_table.cuh:1226-1241; true when this return makes every
   // block of the segment free and this lane won the right to release it
   __device__ bool finish_freeing_block(uint64_t segment, uint64_t nb, uint32_t* control) {
-    atom::fence_release();  // the queue entry is visible before the grant is
+    atom::order();  // the queue entry is visible before the grant is
     int32_t return_id = atom::add(&segs[segment].active, 1);
     if ((int64_t)return_id > (int64_t)nb - 2) count_error_at(control, kErrOverReturn);
     if ((int64_t)return_id == (int64_t)nb - 2) {
