@@ -14,8 +14,10 @@
 //
 // MI355X: the eight XCD L2s are not coherent with each other, so while
 // inserts run, chain pointers and keys are read with agent-scope atomic
-// loads, a new slab's empty keys are published after a release fence, and
-// a slab's key words are read together (one round trip per slab).
+// loads, a new slab's empty keys (agent-scope stores) are performed before
+// the slab is linked (a wait for completion, not a fence: nothing ordinary
+// needs writing back), and a slab's key words are read together (one round
+// trip per slab).
 #pragma once
 
 #include "../allocators/device_utils.hpp"
@@ -51,7 +53,7 @@ struct chaining_table {
     if (!b) return false;
     atom::store((uint64_t*)&b->next, 0ull);
     for (int i = 0; i < S; i++) atom::store((uint64_t*)&b->keys[i], (uint64_t)empty);
-    atom::fence_release();  // the empty keys are visible before the slab is
+    atom::order();  // the empty keys (agent-scope stores) are performed before the slab is
     if (atom::cas((uint64_t*)where, 0ull, (uint64_t)b) != 0ull) alloc->free(b);
     return true;
   }
