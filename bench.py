@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "b59646539728bbe3",
-    "commit": "255d759",
-    "traffic": {"mixed_malloc_kernel": 200416752, "mixed_free_kernel": 190670231},
-    "atomic_requests": {"mixed_malloc_kernel": 280127, "mixed_free_kernel": 534838},
+    "sources_sha": "dcd8dda32b2a5231",
+    "commit": "af56dd6",
+    "traffic": {"mixed_malloc_kernel": 191943661, "mixed_free_kernel": 184404492},
+    "atomic_requests": {"mixed_malloc_kernel": 271951, "mixed_free_kernel": 515206},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
