@@ -226,6 +226,16 @@ __device__ inline void backoff(int iteration) {
   if (iteration > 2) __builtin_amdgcn_s_sleep(2);
 }
 
+// A value that differs between the waves resident on the GPU at one time:
+// the hardware wave slot (HW_ID: wave, SIMD, CU, shader array and engine
+// ids) and the XCD (XCC_ID), mixed with the wall clock.
+__device__ inline uint64_t slot_salt() {
+  const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_REG_HW_ID, 32 bits
+  const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20);   // HW_REG_XCC_ID, bits 3:0
+  return ((uint64_t)(hw & 0xFFFFu) | ((uint64_t)(xcc & 0xFu) << 16) | ((uint64_t)lane_id() << 20)) ^
+         (__builtin_amdgcn_s_memrealtime() << 26);
+}
+
 }  // namespace wave
 
 #endif  // __HIPCC__

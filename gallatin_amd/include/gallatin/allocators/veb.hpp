@@ -16,6 +16,7 @@ namespace allocators {
 constexpr int kMaxVebLayers = 6;           // universes below 2^36 items
 constexpr int kVebHashedAttempts = 15;     // reference VEB_MAX_ATTEMPTS, veb.cuh:59
 constexpr int kSuccessorRestarts = 1 << 16;  // safety bound on stale-summary restarts
+constexpr int kClaimTries = 256;             // lost races inside one word before moving past it
 
 struct veb_tree {
   uint64_t* bits[kMaxVebLayers];    // [0] top summary ... [num_layers-1] items
@@ -35,10 +36,12 @@ struct veb_tree {
 #if defined(__HIPCC__)
   __device__ uint64_t load_word(int layer, uint64_t w) const { return atom::load(&bits[layer][w]); }
 
-  // first set bit strictly above `low` in word `high` of `layer` (-1: any)
-  __device__ int find_next(int layer, uint64_t high, int low) const {
+  // first set bit strictly above `low` in word `high` of `layer` (-1: any);
+  // `seen` (if given) receives the word as loaded
+  __device__ int find_next(int layer, uint64_t high, int low, uint64_t* seen = nullptr) const {
     if (high >= universe[layer]) return -1;
     uint64_t w = load_word(layer, high);
+    if (seen) *seen = w;
     if (low >= 0) w &= ~utils::bitmask((uint64_t)low + 1);
     return utils::ffs64(w) - 1;
   }
@@ -51,12 +54,15 @@ struct veb_tree {
   // bit being cleared, and without a restore its item would stay set but
   // unreachable from the summaries (a free segment no search can find). With
   // one lane the word is still empty and nothing is restored.
-  __device__ bool remove(uint64_t v) {
-    uint64_t high = v >> 6;
-    int low = (int)(v & 63);
+  __device__ bool remove(uint64_t v) { return (take(v >> 6, (int)(v & 63)) >> (v & 63)) & 1ull; }
+
+  // remove() of bit `low` of item word `high`; returns the word as it was
+  // before the clear (the bit was this caller's when it is set there)
+  __device__ uint64_t take(uint64_t high, int low) {
     int layer = num_layers - 1;
     uint64_t old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
-    if (!(old & (1ull << low))) return false;
+    const uint64_t before = old;
+    if (!(old & (1ull << low))) return before;
     while (utils::popc64(old) == 1 && layer > 0) {
       const uint64_t child = high;
       const int child_layer = layer;
@@ -69,7 +75,7 @@ struct veb_tree {
         break;
       }
     }
-    return true;
+    return before;
   }
 
   // set item v; set summary bits upward while a word was empty
@@ -90,13 +96,15 @@ struct veb_tree {
 
   // First item strictly above q reachable through the summaries. A summary
   // bit whose word below turns out empty is stepped over by floating up again.
-  __device__ uint64_t successor_thorough(uint64_t q) const {
+  // `seen` (if given) receives the item word the result was found in, as
+  // loaded.
+  __device__ uint64_t successor_thorough(uint64_t q, uint64_t* seen = nullptr) const {
     uint64_t high = q >> 6;
     int low = (int)(q & 63);
     int layer = num_layers - 1;
     for (int restart = 0; restart < kSuccessorRestarts; restart++) {
       int f;
-      while ((f = find_next(layer, high, low)) == -1) {
+      while ((f = find_next(layer, high, low, seen)) == -1) {
         if (layer == 0) return utils::kFail;
         layer--;
         low = (int)(high & 63);
@@ -105,7 +113,7 @@ struct veb_tree {
       while (layer != num_layers - 1) {
         layer++;
         high = (high << 6) + (uint64_t)f;
-        f = find_next(layer, high, -1);
+        f = find_next(layer, high, -1, seen);
         if (f == -1) break;
       }
       if (f != -1) return (high << 6) + (uint64_t)f;
@@ -114,16 +122,6 @@ struct veb_tree {
       layer--;
       low = (int)(high & 63);
       high >>= 6;
-    }
-    return utils::kFail;
-  }
-
-  __device__ uint64_t lock_offset(uint64_t start) {
-    if (query(start) && remove(start)) return start;
-    for (int i = 0; i < kSuccessorRestarts; i++) {
-      start = successor_thorough(start);
-      if (start == utils::kFail) return start;
-      if (remove(start)) return start;
     }
     return utils::kFail;
   }
@@ -187,11 +185,41 @@ struct veb_tree {
     return top == 0 ? utils::kFail : successor_thorough(0);
   }
 
-  // lowest free item, claimed (reference veb.cuh:979-991)
+  // Lowest free item, claimed (reference veb.cuh:918-937 lock_offset(0),
+  // up to 15 times, veb.cuh:979-991): word 0 read directly (item 0 and the
+  // items above it), then the words above through the summaries. A
+  // candidate is taken with one atomic; a lost race goes on with the word
+  // as that atomic returned it, not with a new search. Once a race is lost,
+  // candidates are set bits picked by a per-wave salt instead of the lowest:
+  // claimers that arrive together (a burst of segment attaches) spread over
+  // the word rather than all trying its lowest bit, one winner per round
+  // trip. A claimer that never loses a race, as a single lane, takes the
+  // lowest free item: the reference's result.
   __device__ uint64_t malloc_first() {
+    const int items = num_layers - 1;
+    bool lost = false;
+    uint64_t salt = 0;
     for (int attempts = 0; attempts < kVebHashedAttempts; attempts++) {
-      uint64_t offset = lock_offset(0);
-      if (offset != utils::kFail) return offset;
+      uint64_t high = 0;
+      uint64_t seen = load_word(items, 0);
+      for (int step = 0; step < kSuccessorRestarts; step++) {
+        uint64_t cand = seen;
+        for (int k = 0; cand != 0 && k < kClaimTries; k++) {
+          int b = utils::ffs64(cand) - 1;
+          if (lost) {
+            salt = utils::splitmix64(salt ? salt : wave::slot_salt());
+            for (uint32_t n = (uint32_t)(salt >> 32) % (uint32_t)utils::popc64(cand); n > 0; n--) cand &= cand - 1;
+            b = utils::ffs64(cand) - 1;
+          }
+          const uint64_t old = take(high, b);
+          if ((old >> b) & 1ull) return high * 64 + (uint64_t)b;
+          lost = true;
+          cand = old;
+        }
+        const uint64_t v = successor_thorough(high * 64 + 63, &seen);
+        if (v == utils::kFail) break;
+        high = v >> 6;
+      }
     }
     return utils::kFail;
   }
