@@ -467,7 +467,8 @@ def global_plain(n_threads, size, seed, verify=GAL_VERIFY_STORE):
 
 def global_replay(ops):
     """Single-lane replay through global_malloc/global_free; returns (results,
-    cu_id): the CU id the lane ran on is the slot the same stream replays with."""
+    cu_id): cu_id is the slot hint of the CU the lane ran on (its XCD in the
+    low three bits), the slot the same stream replays with."""
     arr, n = ops_array(ops)
     out = (ctypes.c_uint64 * max(1, n))()
     cu = ctypes.c_uint32()

@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) batch_kernel(A* a, const uint64_t* sizes,
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint32_t lane = wave::lane_id();
   const uint32_t sub = width_log2 >= 6 ? 0u : lane >> width_log2;
-  const uint32_t hint = __smid() + sub * 0x9E37u;
+  const uint32_t hint = A::cu_hint() + sub * (0x9E37u << 3);
   const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
   const uint64_t span = 64ull * run;
   const uint64_t spans = (n + span - 1) / span;

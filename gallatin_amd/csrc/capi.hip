@@ -149,11 +149,11 @@ __global__ void __launch_bounds__(64) group_replay_kernel(A* a, const gal_group_
 }
 
 // Single-lane replay through the global allocator: global_malloc uses the
-// lane's hardware CU id and thread id 0.
+// lane's CU slot hint (Gallatin::cu_hint) and thread id 0.
 __global__ void global_replay_kernel(const gal_op* ops, uint64_t n, uint64_t* results, uint32_t* status,
                                      uint32_t* cu_id) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  *cu_id = __smid();
+  *cu_id = ga::global_gallatin->cu_hint();
   char* base = ga::global_gallatin->table.memory;
   for (uint64_t i = 0; i < n; i++) {
     const gal_op op = ops[i];

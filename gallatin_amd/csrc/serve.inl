@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256)
       if (claim) {
         const uint32_t left = r.pages - step;
         k = stride ? (left < run ? left : run) : 1u;
-        reserved = k > 1 ? (char*)a->malloc_run_at(r.page_bytes, k, tid, __smid(), 0)
+        reserved = k > 1 ? (char*)a->malloc_run_at(r.page_bytes, k, tid, A::cu_hint(), 0)
                          : (char*)a->malloc(r.page_bytes);
       }
       const uint64_t tb = wall_ticks();

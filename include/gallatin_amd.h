@@ -231,9 +231,10 @@ int gal_free_global_allocator(void);
 int gal_print_global_stats(void);
 /* Single-lane replay through global_malloc / global_free (reference
  * global_allocator.cuh:72-82): one lane of one kernel runs the ops in order
- * (gal_op; slot and tid are ignored: global_malloc uses the lane's hardware
- * CU id and its thread id 0). *cu_id reports the CU id the lane ran on, the
- * slot the same stream replays with through gal_replay or the oracle. */
+ * (gal_op; slot and tid are ignored: global_malloc uses the slot hint of the
+ * lane's CU, its XCD in the low three bits, and its thread id 0). *cu_id
+ * reports that hint, the slot the same stream replays with through
+ * gal_replay or the oracle. */
 int gal_global_replay(const gal_op* ops, uint64_t n, uint64_t* results, uint32_t* cu_id);
 
 /* Global allocator over mapped host memory, and both together. Replace

@@ -159,7 +159,7 @@ int gal_batch_order(gal_batch* b, uint32_t* order, uint64_t cap);
  *                  or GAL_NULL_OFFSET when malloc returned nullptr
  *   GAL_OP_FREE:   arg = byte offset of a live allocation (made by an
  *                  earlier call); results[i] = 0
- * slot and tid are ignored (the kernel uses the hardware CU id and its own
+ * slot and tid are ignored (the kernel uses its CU's slot hint and its own
  * thread ids). Each malloc tags its allocation's first word with a tag of
  * (seed, offset) by atomic exchange (expects 0) and each free checks and
  * clears it: out->tag_errors counts mismatches (a shared allocation). */

@@ -44,7 +44,10 @@ def test_block_churn_rounds_and_fragmentation():
     pool = 64 << 30
     g = ga.Gallatin.generate_on_device(pool, 21)
     try:
-        w = BlockChurn(g, requests=4096, rounds=8, seed=5)
+        # runs are claimed from the top of the segment tree and class segments
+        # from the bottom: the largest run shrinks by about 500 segments a
+        # round and gives out (misses) from about round 8 on
+        w = BlockChurn(g, requests=4096, rounds=11, seed=5)
         for _ in range(w.rounds):
             rec = w.step()
             say(f"[block-churn] {rec}")

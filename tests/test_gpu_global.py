@@ -2,9 +2,10 @@
 global_free / print_global_stats (reference global_allocator.cuh:41-97).
 
 * A single lane through global_malloc/global_free returns the oracle's
-  offsets and leaves the oracle's metadata: global_malloc passes the lane's
-  hardware CU id as the slot hint and thread id 0 to the hashed search, so
-  the oracle replays the stream with that slot and tid 0.
+  offsets and leaves the oracle's metadata: global_malloc passes its CU's
+  slot hint (the CU id with its XCD in the low three bits) and thread id 0
+  to the hashed search, so the oracle replays the stream with that slot and
+  tid 0.
 * The reference's global test kernel shape at its full size (1 GiB pool,
   2^20 threads x global_malloc(16 B), tag, check, global_free): no miss, no
   clash, quiescent metadata after.
@@ -57,7 +58,7 @@ def global_stream(seed, n, sizes, free_p):
 def test_global_replay_bit_exact(sizes):
     seed = 57
     ops = global_stream(seed, 6000, sizes, 0.4)
-    # the CU id the lane will run on is known only afterwards; the stream is
+    # the CU the lane will run on is known only afterwards; the stream is
     # valid (every free names a malloc that succeeded) for every slot hint
     # modulo 128, the largest slot count (replay raises otherwise)
     for slot in range(128):
@@ -66,7 +67,7 @@ def test_global_replay_bit_exact(sizes):
     try:
         dev, _ = ga.global_handles()
         got, cu = ga.global_replay(ops)
-        say(f"[global-replay] {len(ops)} ops on CU id {cu}")
+        say(f"[global-replay] {len(ops)} ops with CU slot hint {cu}")
         # the same stream through the oracle, with the slot the lane ran on
         o = oracle.Oracle(1 << 30, seed)
         expect = o.replay([(k, a, 0, cu) for (k, a, _, _) in ops])
