@@ -693,39 +693,31 @@ struct Gallatin {
     return (uint64_t)block * kSlicesPerBlock;
   }
 
-  // contiguous segments under the segment lock (reference :790-826). The
-  // calling lanes (`lanes`, all active, the call uniform) find the word the
-  // claim starts at together (veb_tree::gather_plan), which reads the tree
-  // one word per lane per round trip; one lane holds the lock and claims.
-  // With one lane the result is the reference's.
+  // contiguous segments (reference :790-826). The calling lanes (`lanes`,
+  // all active, the call uniform) find the word the claim starts at
+  // together (veb_tree::gather_plan), which reads the tree one word per lane
+  // per round trip; one lane then claims the run from that word with the
+  // gather rule (veb_tree::gather_multiple). When the tree as read holds no
+  // run of n free segments the request fails at once (`oom`, a further
+  // attempt would repeat it).
   //
-  // The same search first runs without the lock: when the tree as read now
-  // holds no run of n free segments the request fails at once (`oom`, a
-  // further attempt would repeat it). The lock orders claimers only (frees
-  // do not take it), and a run that another claimer holds for a moment is
-  // claimed at that moment, so this is a failure the locked search could
-  // also give; with one lane the tree is the same either way and the locked
-  // search fails without changing anything. Under heavy out-of-memory
-  // traffic failing requests no longer queue for the lock.
+  // The reference serialises these claims on a spin lock (bit T of its lock
+  // word). Here they run concurrently: every bit of a run is taken with an
+  // atomic and a claim that finds part of its run gone puts back exactly
+  // what it took and goes on searching (gather_bottom), so two claims never
+  // share a segment; the lock added nothing to that, and with it every
+  // claim of the GPU waited for the one before it (the block and segment
+  // churn ran at 7e5 mallocs+frees/s). What concurrency changes is which run
+  // a claim gets: the highest run when its search ran, not at one
+  // serialised instant. With one lane nothing runs between the search and
+  // the claim, and the result is the reference's.
   __device__ __noinline__ uint64_t malloc_segment_allocation(uint64_t n, uint64_t lanes, bool& oom) {
-    oom = segment_tree.gather_plan(n, lanes) == utils::kFail;
+    const uint64_t start = segment_tree.gather_plan(n, lanes);
+    oom = start == utils::kFail;
     if (oom) return utils::kFail;
     const int leader = wave::first_lane(lanes);
-    uint32_t locked = 0;
-    if ((int)wave::lane_id() == leader) {
-      for (int spin = 0; spin <= (1 << 22); spin++) {
-        if (acquire_tree_lock(kNumTrees)) {
-          locked = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    if (!wave::readlane(locked, leader)) return utils::kFail;
-    atom::order();
-    const uint64_t start = segment_tree.gather_plan(n, lanes);
     uint64_t first = utils::kFail;
-    if ((int)wave::lane_id() == leader && start != utils::kFail) {
+    if ((int)wave::lane_id() == leader) {
       first = segment_tree.gather_multiple(n, start);
       if (first != utils::kFail) {
         atom::sub(ctl_word(control, kCtlFreeSegs), (uint32_t)n);
@@ -734,7 +726,6 @@ struct Gallatin {
     }
     first = wave::readlane64(first, leader);
     atom::order();
-    if ((int)wave::lane_id() == leader) release_tree_lock(kNumTrees);
     return first == utils::kFail ? utils::kFail : first * kBlocksPerSegment * kSlicesPerBlock;
   }
 
@@ -744,16 +735,32 @@ struct Gallatin {
     return table.memory + (offset / per) * bytes_per_segment + (offset % per) * table_type::tree_alloc_size(tree);
   }
 
-  // Block and segment requests, one request of the calling lanes at a time:
-  // a block request runs in its own lane; a segment request runs with every
-  // calling lane helping to search the segment tree. Kept out of line: the
-  // slice path inlined into a kernel then saves no registers for it.
+  // Block and segment requests. A segment request runs with every calling
+  // lane helping to search the segment tree, one request of the calling
+  // lanes at a time. A block request runs in its own lane: one at a time in
+  // lane order as well when `spread` is false (the order the deterministic
+  // multi-lane replays model), all of the wave's block requests together
+  // under load (their lanes run the block search side by side: with one at
+  // a time a wave of block requests took as long as all of them in a row).
+  // Kept out of line: the slice path inlined into a kernel then saves no
+  // registers for it.
+  template <bool spread>
   __device__ __noinline__ void* malloc_large(uint64_t size, uint64_t tid, int block_tree) {
     const bool is_block = (uint32_t)block_tree < kNumTrees;
     const uint64_t n = is_block ? 0 : (size - 1) / bytes_per_segment + 1;
     const uint64_t callers = wave::active_mask();
     void* out = nullptr;
-    for (uint64_t pending = callers; pending; pending &= pending - 1) {
+    uint64_t pending = callers;
+    if (spread) {
+      if (is_block) {
+        uint64_t offset = utils::kFail;
+        for (int a = 0; a < kBlockAttempts && offset == utils::kFail; a++)
+          offset = malloc_block_allocation((uint32_t)block_tree, tid);
+        if (offset != utils::kFail) out = offset_to_allocation(offset, (uint32_t)block_tree);
+      }
+      pending &= ~wave::ballot(is_block);
+    }
+    for (; pending; pending &= pending - 1) {
       const int who = wave::first_lane(pending);
       if (wave::readlane((uint32_t)is_block, who)) {
         if ((int)wave::lane_id() == who) {
@@ -786,7 +793,7 @@ struct Gallatin {
     uint32_t count = 1;
     if (tree >= kNumTrees) {
       const int block_tree = utils::get_first_bit_bigger(size) - kSmallestTreeBits;
-      if (block_tree >= 0) return malloc_large(size, tid, block_tree);
+      if (block_tree >= 0) return malloc_large<spread>(size, tid, block_tree);
       count = 1u << (tree - (kNumTrees - 1));
       tree = kNumTrees - 1;
     }

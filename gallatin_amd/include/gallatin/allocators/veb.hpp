@@ -70,6 +70,11 @@ struct veb_tree {
       low = (int)(high & 63);
       high >>= 6;
       old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
+      // The word below is read only once the clear is performed: read
+      // earlier, it can miss an insert whose summary update came before the
+      // clear (the insert then saw the bit still set and stopped), and the
+      // word would keep an item that no search can reach.
+      atom::order();
       if (atom::load(&bits[child_layer][child]) != 0) {
         atom::fetch_or(&bits[layer][high], 1ull << low);
         break;
@@ -291,11 +296,15 @@ struct veb_tree {
   // words and the high bits of the next lower word (reference veb.cuh:323-452).
   // A run may not extend below word 0 and never starts in word 0. `from`
   // starts the search at a lower word (one that gather_plan found); attempts
-  // above it would have failed and left the words as they were.
+  // above it would have failed and left the words as they were. A claim
+  // that finds part of a run gone (taken by a concurrent claim, or held for
+  // a moment by one that is putting it back) restores what it took and
+  // tries again, at most kClaimTries times in all (steps past words without
+  // a run are not counted).
   __device__ uint64_t gather_bottom(uint64_t n, uint64_t from = ~0ull) {
     uint64_t* b = bits[num_layers - 1];
     uint64_t w = from < words[num_layers - 1] ? from : words[num_layers - 1] - 1;
-    while (w != 0) {
+    for (int lost = 0; w != 0 && lost < kClaimTries;) {
       uint64_t need = n;
       int run = utils::trailing_ones(atom::load(&b[w]));
       if (run == 0) {
@@ -307,12 +316,14 @@ struct veb_tree {
         uint64_t got = atom::fetch_and(&b[w], ~mask);
         if ((uint64_t)utils::popc64(got & mask) == need) return w * 64 + (run - need);
         atom::fetch_or(&b[w], got & mask);
+        lost++;
         continue;
       }
       uint64_t low_mask = utils::bitmask(run);
       uint64_t got = atom::fetch_and(&b[w], ~low_mask);
       if (utils::popc64(got & low_mask) != run) {
         atom::fetch_or(&b[w], got & low_mask);
+        lost++;
         continue;
       }
       need -= run;
@@ -332,7 +343,10 @@ struct veb_tree {
         }
         need -= 64;
       }
-      if (!ok) continue;
+      if (!ok) {
+        lost++;
+        continue;
+      }
       if (w == 0) {
         rollback(w, n - need);
         return utils::kFail;
@@ -343,6 +357,7 @@ struct veb_tree {
       if ((uint64_t)utils::popc64(got & top_mask) == need) return w * 64 + 64 - need;
       atom::fetch_or(&b[w], got & top_mask);
       rollback(w + 1, n - need);
+      lost++;
     }
     return utils::kFail;
   }
@@ -361,7 +376,9 @@ struct veb_tree {
         if (atom::load(&bits[layer + 1][w]) != 0) continue;
         atom::fetch_and(&bits[layer][w >> 6], ~(1ull << (w & 63)));
         // an insert into the word between the check and the clear set this
-        // very bit (it saw an empty word): put it back (as remove does)
+        // very bit (it saw an empty word): put it back (as remove does,
+        // reading the word only once the clear is performed)
+        atom::order();
         if (atom::load(&bits[layer + 1][w]) != 0) atom::fetch_or(&bits[layer][w >> 6], 1ull << (w & 63));
       }
     }

@@ -35,7 +35,9 @@ def expected_order(sizes, min_size):
 
 
 def test_batch_binning_and_batched_malloc_free():
-    pool, seed = 2 << 30, 5
+    # 4 GiB: the classes' segments (taken from the bottom of the segment tree)
+    # leave the top words whole for the three segment runs
+    pool, seed = 4 << 30, 5
     rng = np.random.default_rng(11)
     n = 70001  # not a multiple of the binning chunk
     sizes = (16 << rng.integers(0, 9, n)).astype(np.uint64)
