@@ -168,6 +168,93 @@ struct veb_tree {
     return utils::kFail;
   }
 
+  // find_random_valid_index of a two-layer tree (at most 4096 items: pools
+  // up to 32 GiB at 16 MiB segments): the same attempts in the same order,
+  // with the same result on an unchanged tree, but the top word is read once
+  // and each item word at most once (a few kept in registers), and the
+  // highest item's word and the first hashed start's word are read in one
+  // round trip. Written out, the general search read the top word three
+  // times and re-read the start's word for the successor step: five to
+  // seven dependent loads per block request under load, now two or three.
+  __device__ uint64_t find_random_valid_index_2(uint64_t tid) const {
+    const uint64_t* items = bits[1];
+    const uint64_t top = load_word(0, 0);
+    if (top == 0) return utils::kFail;  // looks_empty
+    const uint64_t hash1 = utils::murmur64a(tid, seed);
+    const uint64_t hash2 = utils::murmur64a(tid, hash1);
+    // the item words read last (index, value; ~0: none), newest first; only
+    // constant indices, so that they stay in registers
+    constexpr int kKept = 4;
+    uint64_t kept_w[kKept], kept_v[kKept];
+#pragma unroll
+    for (int i = 0; i < kKept; i++) kept_w[i] = kept_v[i] = ~0ull;
+    auto keep = [&](uint64_t w, uint64_t v) {
+#pragma unroll
+      for (int i = kKept - 1; i > 0; i--) {
+        kept_w[i] = kept_w[i - 1];
+        kept_v[i] = kept_v[i - 1];
+      }
+      kept_w[0] = w;
+      kept_v[0] = v;
+    };
+    auto get = [&](uint64_t w) -> uint64_t {
+      uint64_t v = 0;
+      bool hit = false;
+#pragma unroll
+      for (int i = 0; i < kKept; i++)
+        if (!hit && kept_w[i] == w) {
+          v = kept_v[i];
+          hit = true;
+        }
+      if (!hit) {
+        v = atom::load(&items[w]);
+        keep(w, v);
+      }
+      return v;
+    };
+    {
+      const uint64_t wb = (uint64_t)(63 - utils::clz64(top));
+      const uint64_t w0 = (hash1 % total_universe) >> 6;
+      const uint64_t vb = atom::load(&items[wb]);
+      const uint64_t v0 = w0 != wb ? atom::load(&items[w0]) : vb;
+      keep(wb, vb);
+      if (w0 != wb) keep(w0, v0);
+    }
+    // max_item: the highest summary bit whose word holds an item
+    uint64_t top_item = utils::kFail;
+    for (uint64_t sum = top; sum != 0;) {
+      const uint64_t w = (uint64_t)(63 - utils::clz64(sum));
+      const uint64_t v = get(w);
+      if (v != 0) {
+        top_item = w * 64 + (uint64_t)(63 - utils::clz64(v));
+        break;
+      }
+      sum &= ~(1ull << w);  // stale summary bit: back up below it
+    }
+    if (top_item == utils::kFail) return utils::kFail;
+    // successor_thorough: first item strictly above q reachable through the
+    // summaries (a stale summary bit is stepped over)
+    auto successor = [&](uint64_t q) -> uint64_t {
+      const uint64_t w = q >> 6;
+      const uint64_t above = get(w) & ~utils::bitmask((q & 63) + 1);
+      if (above != 0) return w * 64 + (uint64_t)(utils::ffs64(above) - 1);
+      for (uint64_t sum = top & ~utils::bitmask(w + 1); sum != 0; sum &= sum - 1) {
+        const uint64_t w2 = (uint64_t)(utils::ffs64(sum) - 1);
+        const uint64_t v = get(w2);
+        if (v != 0) return w2 * 64 + (uint64_t)(utils::ffs64(v) - 1);
+      }
+      return utils::kFail;
+    };
+    for (int attempts = 0; attempts < kVebHashedAttempts; attempts++) {
+      const uint64_t start = (hash1 + (uint64_t)attempts * hash2) % total_universe;
+      if (start > top_item) continue;
+      if ((get(start >> 6) >> (start & 63)) & 1ull) return start;  // query
+      const uint64_t index = successor(start);
+      if (index != utils::kFail) return index;
+    }
+    return top_item == 0 ? utils::kFail : successor(0);
+  }
+
   // Hashed start positions (reference veb.cuh:946-977); tid is the caller's
   // global thread id. An attempt succeeds exactly when some item lies at or
   // above its start, so starts above the highest item are skipped without
@@ -175,6 +262,7 @@ struct veb_tree {
   // tree unchanged during the call the result is the reference's; a sparse
   // tree no longer costs a failed successor walk per hashed start.
   __device__ uint64_t find_random_valid_index(uint64_t tid) const {
+    if (num_layers == 2) return find_random_valid_index_2(tid);
     if (looks_empty()) return utils::kFail;
     const uint64_t top = max_item();
     if (top == utils::kFail) return utils::kFail;
