@@ -334,7 +334,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads (forms, serving, blocks, slab)")
     ap.add_argument("--serving-requests", type=int, default=1000000, help="requests of the serving workload")
     ap.add_argument("--serving-rate", type=float, default=50000.0)
-    ap.add_argument("--cpu-threads-log2", type=int, default=22, help="oracle sample size (threads per round)")
+    ap.add_argument("--cpu-threads-log2", type=int, default=21,
+                    help="oracle sample size (threads per round; 2^21 is about 20 s of one core)")
     ap.add_argument(
         "--copy-calibration",
         type=int,
