@@ -122,6 +122,14 @@ template <class T>
 __device__ inline T exch(T* p, same_t<T> v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The word's current value, read with a read-modify-write that changes
+// nothing: it is performed where the word's atomics are (beyond the XCD
+// L2s), so it sees every atomic performed on the word before it. For the
+// few reads whose result nothing else validates.
+template <class T>
+__device__ inline T read_current(T* p) {
+  return __hip_atomic_fetch_or(p, (T)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // returns the value found (== expected on success), like CUDA atomicCAS
 template <class T>
 __device__ inline T cas(T* p, same_t<T> expected, same_t<T> desired) {

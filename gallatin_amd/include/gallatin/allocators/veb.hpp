@@ -70,12 +70,13 @@ struct veb_tree {
       low = (int)(high & 63);
       high >>= 6;
       old = atom::fetch_and(&bits[layer][high], ~(1ull << low));
-      // The word below is read only once the clear is performed: read
-      // earlier, it can miss an insert whose summary update came before the
-      // clear (the insert then saw the bit still set and stopped), and the
-      // word would keep an item that no search can reach.
+      // The word below is read only once the clear is performed, and with
+      // an atomic (read_current): read earlier, or served from a copy the
+      // XCD's L2 kept, it can miss an insert whose summary update came
+      // before the clear (the insert then saw the bit still set and
+      // stopped), and the word would keep an item that no search can reach.
       atom::order();
-      if (atom::load(&bits[child_layer][child]) != 0) {
+      if (atom::read_current(&bits[child_layer][child]) != 0) {
         atom::fetch_or(&bits[layer][high], 1ull << low);
         break;
       }
@@ -465,9 +466,9 @@ struct veb_tree {
         atom::fetch_and(&bits[layer][w >> 6], ~(1ull << (w & 63)));
         // an insert into the word between the check and the clear set this
         // very bit (it saw an empty word): put it back (as remove does,
-        // reading the word only once the clear is performed)
+        // reading the word with an atomic once the clear is performed)
         atom::order();
-        if (atom::load(&bits[layer + 1][w]) != 0) atom::fetch_or(&bits[layer][w >> 6], 1ull << (w & 63));
+        if (atom::read_current(&bits[layer + 1][w]) != 0) atom::fetch_or(&bits[layer][w >> 6], 1ull << (w & 63));
       }
     }
     return found;
