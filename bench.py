@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "dcd8dda32b2a5231",
-    "commit": "af56dd6",
-    "traffic": {"mixed_malloc_kernel": 191943661, "mixed_free_kernel": 184404492},
-    "atomic_requests": {"mixed_malloc_kernel": 271951, "mixed_free_kernel": 515206},
+    "sources_sha": "2360b626b5438a91",
+    "commit": "1523616",
+    "traffic": {"mixed_malloc_kernel": 194597123, "mixed_free_kernel": 184566548},
+    "atomic_requests": {"mixed_malloc_kernel": 335529, "mixed_free_kernel": 585014},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
@@ -334,8 +334,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads (forms, serving, blocks, slab)")
     ap.add_argument("--serving-requests", type=int, default=1000000, help="requests of the serving workload")
     ap.add_argument("--serving-rate", type=float, default=50000.0)
-    ap.add_argument("--cpu-threads-log2", type=int, default=21,
-                    help="oracle sample size (threads per round; 2^21 is about 20 s of one core)")
+    ap.add_argument("--cpu-threads-log2", type=int, default=22,
+                    help="oracle sample size (threads per round; the workload's own 2^22 is about 40 s of one core)")
     ap.add_argument(
         "--copy-calibration",
         type=int,
