@@ -106,7 +106,7 @@ struct Gallatin {
   static constexpr int kSmallestBits = ilog2(smallest);
   static constexpr int kSmallestTreeBits = ilog2(smallest * 4096);
   static constexpr uint64_t kBlocksPerSegment = table_type::blocks_per_segment;
-  static_assert(kNumTrees <= 30, "class lock bits and the segment lock fit one u32");
+  static_assert(kNumTrees <= 30, "the class lock bits fit one u32");
 
   static constexpr uint32_t total_slots() {
     uint32_t t = 0;

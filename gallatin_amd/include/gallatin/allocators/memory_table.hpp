@@ -58,7 +58,7 @@ struct segment_counters {
 
 // device error counters (reference traps), kept in the control word array
 enum control_word : uint32_t {
-  kCtlLocks = 0,          // bit c: class c is attaching a segment; bit T: segment claim
+  kCtlLocks = 0,          // bit c: class c is attaching a segment
   kCtlErrors = 1,         // internal inconsistencies observed (sum of the sites below)
   kCtlBadFrees = 2,       // frees of pointers the allocator does not own
   kCtlBootStatus = 3,     // 0 booting, 1 ok, 2 failed
