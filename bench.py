@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "2360b626b5438a91",
-    "commit": "1523616",
-    "traffic": {"mixed_malloc_kernel": 194597123, "mixed_free_kernel": 184566548},
-    "atomic_requests": {"mixed_malloc_kernel": 335529, "mixed_free_kernel": 585014},
+    "sources_sha": "41b4c52120a2e578",
+    "commit": "a296506",
+    "traffic": {"mixed_malloc_kernel": 193316387, "mixed_free_kernel": 184413125},
+    "atomic_requests": {"mixed_malloc_kernel": 268255, "mixed_free_kernel": 476325},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
