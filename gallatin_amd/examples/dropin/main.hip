@@ -39,8 +39,11 @@ __global__ void roundtrip(A* local, uint64_t n, unsigned long long* counts) {
 
 int main() {
   using local_type = Gallatin<16ull * 1024 * 1024, 16ull, 4096ull>;
-  init_global_allocator(1ull << 30, 42, false);
-  local_type* local = local_type::generate_on_device(1ull << 30, 43, false);
+  // 4 GiB each: 2^20 threads per launch claim and free at once, and a 1 GiB
+  // pool (64 segments for nine classes) could run a class out of blocks for
+  // a moment while its spent blocks wait for their last frees
+  init_global_allocator(4ull << 30, 42, false);
+  local_type* local = local_type::generate_on_device(4ull << 30, 43, false);
   unsigned long long* counts = nullptr;
   if (!local || hipMalloc(&counts, 16) != hipSuccess || hipMemset(counts, 0, 16) != hipSuccess) {
     printf("dropin: setup failed\n");

@@ -125,10 +125,14 @@ __device__ inline T exch(T* p, same_t<T> v) {
 // The word's current value, read with a read-modify-write that changes
 // nothing: it is performed where the word's atomics are (beyond the XCD
 // L2s), so it sees every atomic performed on the word before it. For the
-// few reads whose result nothing else validates.
+// few reads whose result nothing else validates. The zero it ORs in is
+// opaque to the compiler, which otherwise turns an OR with a known zero into
+// a plain agent-scope load.
 template <class T>
 __device__ inline T read_current(T* p) {
-  return __hip_atomic_fetch_or(p, (T)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t zero;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+  return __hip_atomic_fetch_or(p, (T)zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // returns the value found (== expected on success), like CUDA atomicCAS
 template <class T>

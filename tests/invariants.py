@@ -102,11 +102,14 @@ def check_quiescent(image, seg_bytes, min_size):
         assert all(chunk[x] == c for x in listed), f"class {c} tree lists a segment of another class"
         # every listed segment is reachable: a word that holds items has its
         # summary bit set (class trees only change by single inserts/removes)
-        for upper, lower in zip(layers[:-1], layers[1:]):
+        for level, (upper, lower) in enumerate(zip(layers[:-1], layers[1:])):
             summary = set(bits_of(upper))
-            holding = [w for w, val in enumerate(np.asarray(lower, dtype=np.uint64).tolist()) if val]
+            lower_vals = np.asarray(lower, dtype=np.uint64).tolist()
+            holding = [w for w, val in enumerate(lower_vals) if val]
             lost = [w for w in holding if w not in summary]
-            assert not lost, f"class {c} tree: words {lost[:8]} hold items but their summary bits are clear"
+            assert not lost, (f"class {c} tree: words {lost[:8]} of layer {level + 1} hold items "
+                              f"({[hex(lower_vals[w]) for w in lost[:4]]}) but their summary bits in layer "
+                              f"{level} are clear (layer {level}: {[hex(int(x)) for x in np.asarray(upper)[:8]]})")
 
     slots = s["slots"].astype(np.int64)
     pinned = slots[slots > 0] - 1
