@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "41b4c52120a2e578",
-    "commit": "a296506",
-    "traffic": {"mixed_malloc_kernel": 193316387, "mixed_free_kernel": 184413125},
-    "atomic_requests": {"mixed_malloc_kernel": 268255, "mixed_free_kernel": 476325},
+    "sources_sha": "a5aef2ae49eac96c",
+    "commit": "ddab923",
+    "traffic": {"mixed_malloc_kernel": 196045548, "mixed_free_kernel": 197670632},
+    "atomic_requests": {"mixed_malloc_kernel": 272433, "mixed_free_kernel": 488515},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
@@ -177,7 +177,7 @@ def forms(seed):
     g = ga.Gallatin.generate_on_device(200 << 30, seed)
     try:
         for name, med, cap, n, slots, window in (("serving", 1024, 32768, 1 << 19, 65536, 0.1),
-                                                  ("serving_short", 16, 1024, 1 << 23, 65536, 0.05)):
+                                                  ("serving_short", 16, 1024, 1 << 23, 65536, 0.04)):
             reqs = ga.generate_requests(n, 0.0, median_pages=med, sigma=1.0, max_pages=cap, page_bytes=0,
                                         size_classes=9, seed=seed)
             with ga.Serving(g, len(reqs)) as s:
