@@ -231,6 +231,8 @@ class Gallatin:
         "recycle",
         "replace_ticks16",
         "refill_ticks16",
+        "ahead",
+        "ahead_taken",
     )
 
     def stats(self):

@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(64) group_replay_kernel(A* a, const gal_group_
     if (!in) {
       results[i * 64 + lane] = GAL_NULL_OFFSET;
     } else if (is_malloc) {
-      void* p = kind == GAL_OP_MALLOC_SPREAD ? a->template malloc_at<true>(arg, op->tid[lane], op->slot)
+      // the spread form as the oracle models it: first look, no block fetched ahead
+      void* p = kind == GAL_OP_MALLOC_SPREAD ? a->template malloc_at<true, false>(arg, op->tid[lane], op->slot)
                                              : a->template malloc_at<false>(arg, op->tid[lane], op->slot);
       results[i * 64 + lane] = p ? (uint64_t)((char*)p - base) : GAL_NULL_OFFSET;
     } else {
@@ -282,7 +283,7 @@ struct impl {
     uint64_t words = 8;
     for (int l = 0; l < host.segment_tree.num_layers; l++) words += host.segment_tree.words[l] * (1 + A::kNumTrees);
     const uint64_t nseg = i.num_segments, nblocks = nseg * A::kBlocksPerSegment;
-    uint64_t bytes = words * 8 + ((nseg * 2 + 7) / 8) * 8 + nblocks * 12 + nseg * 16 + i.total_slots * 4;
+    uint64_t bytes = words * 8 + ((nseg * 2 + 7) / 8) * 8 + nblocks * 12 + nseg * 16 + i.total_slots * 8;
     i.snapshot_bytes = ((bytes + 7) / 8) * 8;
     i.metadata_bytes =
         words * 8 + nseg * 4 + nblocks * (sizeof(ga::Block) + 4) + nseg * 16 + i.total_slots * ga::kSlotStride * 4 +
@@ -340,13 +341,18 @@ struct impl {
     if (hipMemcpy(p, host.table.segs, nseg * 16, hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
     p += nseg * 16;
     {
-      // one entry per 128-B line on the device, packed in the image
-      std::vector<uint32_t> lines((size_t)A::total_slots() * ga::kSlotStride);
+      // each slot's entry and ahead word have 128-B lines of their own on the
+      // device; packed in the image, the entries then the ahead words
+      const uint32_t n = A::total_slots();
+      std::vector<uint32_t> lines((size_t)n * ga::kSlotStride);
       if (hipMemcpy(lines.data(), host.slots, lines.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
         return GAL_ERR_HIP;
-      for (uint32_t k = 0; k < A::total_slots(); k++) std::memcpy(p + 4 * k, &lines[(size_t)k * ga::kSlotStride], 4);
+      for (uint32_t k = 0; k < n; k++) {
+        std::memcpy(p + 4 * (size_t)k, &lines[(size_t)k * ga::kSlotStride], 4);
+        std::memcpy(p + 4 * ((size_t)n + k), &lines[(size_t)k * ga::kSlotStride + ga::kAheadWord], 4);
+      }
     }
-    p += A::total_slots() * 4;
+    p += A::total_slots() * 8;
     uint64_t used = (uint64_t)(p - out);
     if (h->info.snapshot_bytes > used) std::memset(p, 0, h->info.snapshot_bytes - used);
     return GAL_OK;

@@ -234,18 +234,24 @@ __device__ inline uint64_t wall_ticks() { return __builtin_amdgcn_s_memrealtime(
 template <class A>
 __global__ void mixed_malloc_kernel(A* a, uint64_t* live, uint64_t base, uint64_t n, uint32_t round, uint64_t seed,
                                     uint32_t verify, run_counters* c, uint32_t* lat) {
-  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  // 32 bits: a round has fewer than 2^32 threads (gal_mixed_reset). After
+  // malloc the index is rebuilt from the wave's first index (a scalar) and
+  // the lane id: kept in a vector register across the call, it made the
+  // kernel spill one (1-D blocks of whole waves: lane = threadIdx.x % 64).
+  const uint32_t tid = threadIdx.x + blockIdx.x * blockDim.x;
+  const uint32_t wave_first = __builtin_amdgcn_readfirstlane(tid) & ~63u;
   bool got = false, bad = false, miss = false;
   if (tid < n) {
     const uint64_t t0 = wall_ticks();
     uint64_t* p = (uint64_t*)a->malloc(gw::mixed_size(seed, round, tid));
     const uint64_t t1 = wall_ticks();
+    const uint32_t me = wave_first + wave::lane_id();
     // one latency sample per wave: the wave's time inside malloc
-    if (lat && (tid & 63) == 0) lat[(base + tid) >> 6] = (uint32_t)(t1 - t0);
-    live[tid] = (uint64_t)p;
+    if (lat && (me & 63) == 0) lat[(base + me) >> 6] = (uint32_t)(t1 - t0);
+    live[me] = (uint64_t)p;
     if (p) {
       got = true;
-      bad = !tag_in(p, gw::payload_tag(seed, base + tid), verify);
+      bad = !tag_in(p, gw::payload_tag(seed, base + me), verify);
     } else {
       miss = true;
     }
@@ -609,8 +615,9 @@ struct wl_impl {
   }
 
   // Memory the admission rule keeps back for the classes the page sizes
-  // claim from: every slot of such a class holding a partly filled block,
-  // two segments per class attached and not yet used up, and, when a class
+  // claim from: every slot of such a class holding a partly filled block and
+  // a block fetched ahead, two segments per class attached and not yet used
+  // up, and, when a class
   // has several blocks per segment, a tenth of the pool for recycled blocks
   // of segments not listed again yet (a segment is listed again once a
   // tenth of its blocks are back). Pages of the segment path need none.
@@ -636,7 +643,7 @@ struct wl_impl {
       if (!used[c]) continue;
       if (slots_used[c]) {
         const uint64_t block = A::table_type::tree_alloc_size(c) * ga::kSlicesPerBlock;
-        bytes += (uint64_t)ga::usable_slots_of(c, h->info.num_segments * S, block) * block;
+        bytes += 2 * (uint64_t)ga::usable_slots_of(c, h->info.num_segments * S, block) * block;
       }
       bytes += 2 * S;
       hidden |= A::table_type::blocks_in_segment(c) > 1;

@@ -61,7 +61,18 @@ constexpr uint32_t kPinnedSlotsMin = 4;     // MIN_PINNED_CUTOFF
 // hold at most 1/kSlotShare of the pool in blocks (at least its pinned slots),
 // so that a small pool is not tied up in partly used blocks.
 constexpr uint32_t kSlotsUnderLoad = 64;
-constexpr uint32_t kSlotStride = 32;  // u32 words between slot entries: one 128-B line each
+// u32 words between slot entries: two 128-B lines per slot, the slot's entry
+// (word 0) and the block fetched ahead for it (word kAheadWord)
+constexpr uint32_t kSlotStride = 64;
+constexpr uint32_t kAheadWord = 32;
+// Under load, the group whose claim crosses this slice of a slot's block
+// fetches the slot's next block ahead of time (see fetch_ahead).
+constexpr uint32_t kAheadAt = 2048;
+// ORed into the class argument of replace_block: only fetch the slot's next
+// block (the slot's block is not spent). A flag bit rather than a second
+// call site or one more argument, either of which made the claim path spill
+// a register.
+constexpr uint32_t kFetchOnly = 1u << 31;
 constexpr uint64_t kSlotShare = 16;
 
 // Where the pool lives (reference memory_table.cuh:61). Metadata always lives
@@ -157,6 +168,11 @@ struct Gallatin {
 
   __device__ uint32_t* slot_ptr(uint32_t tree, uint32_t index) const {
     return &slots[(uint64_t)(slot_offset[tree] + index) * kSlotStride];
+  }
+  // the block fetched ahead for a slot: 0 none, block id + 1, or kSlotBusy
+  // while one lane fetches it
+  __device__ uint32_t* ahead_ptr(uint32_t tree, uint32_t index) const {
+    return &slots[(uint64_t)(slot_offset[tree] + index) * kSlotStride + kAheadWord];
   }
   __device__ int32_t* empty_pinned_word(uint32_t tree) const { return class_words + tree * kCtlStride; }
 
@@ -314,11 +330,94 @@ struct Gallatin {
     free_block(block);
   }
 
+  // Blocks fetched ahead (no counterpart in the reference, whose slot is
+  // busy for the whole block search of a replacement). Under load, the group
+  // whose claim crosses slice kAheadAt of a slot's block fetches the slot's
+  // next block into the slot's ahead word while the slot keeps serving; the
+  // replacement that finds the block spent publishes that block (the slot is
+  // busy for a few memory round trips instead of a search) and fetches the
+  // next one. An ahead word changes 0 -> busy (the one lane that fetches) ->
+  // block + 1 or 0 (that lane) -> 0 (the lane whose compare-and-swap takes
+  // it), so a block fetched ahead is published once. A replacement that
+  // finds no memory takes the block fetched ahead for another slot of the
+  // class before it leaves its slot empty, so those blocks are spent before
+  // the class runs dry. One lane never fetches ahead, so single-lane results
+  // are the reference's (their ahead words stay 0).
+  //
+  // Take the block `seen` in slot (tree, index)'s ahead word; 0 if it is not
+  // a block or another lane took it first.
+  __device__ uint32_t take_ahead(uint32_t tree, uint32_t index, uint32_t seen) {
+    if (seen == 0u || seen == kSlotBusy || atom::cas(ahead_ptr(tree, index), seen, 0u) != seen) return 0u;
+    count_stat(control, kStatAheadTaken);
+    return seen;
+  }
+
+  // A block fetched ahead for any usable slot of the class, taken; 0 if none.
+  // For a slot about to be left empty: the class has no other memory.
+  __device__ uint32_t take_any_ahead(uint32_t tree) {
+    const uint32_t n = usable_slots[tree];
+    for (uint32_t base = 0; base < n; base += 8) {
+      uint32_t seen[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++) seen[k] = base + k < n ? atom::load(ahead_ptr(tree, base + k)) : 0u;
+      for (uint32_t k = 0; k < 8; k++) {
+        const uint32_t got = take_ahead(tree, base + k, seen[k]);
+        if (got != 0u) return got;
+      }
+    }
+    return 0u;
+  }
+
+  // Fetch the next block of slot (tree, index) ahead of time, unless one is
+  // ready or being fetched. A slot that ran empty meanwhile (its replacement
+  // found no block before this one arrived) gets the block directly.
+  __device__ __noinline__ void fetch_ahead(uint32_t tree, uint32_t index, uint64_t tid) {
+    // No free segment and nothing listed in the class's tree: the search
+    // would fail.
+    const uint64_t top = sub_trees[tree].top_word();
+    if (atom::load(ctl_word(control, kCtlFreeSegs)) == 0u && sub_trees[tree].top_looks_empty(top)) return;
+    uint32_t* a = ahead_ptr(tree, index);
+    if (atom::cas(a, 0u, kSlotBusy) != 0u) return;
+    count_stat(control, kStatAhead);
+    bool last = false;
+    const int64_t fresh = request_new_block_from_tree(tree, tid, last);
+    atom::order();  // the block's counter reset precedes its publication
+    bool placed = false;
+    uint32_t* s = slot_ptr(tree, index);
+    if (fresh >= 0 && atom::load(s) == 0u) {
+      const bool pinned = index < pinned_slots_of(tree);
+      if (pinned) atom::sub(empty_pinned_word(tree), 1);
+      placed = atom::cas(s, 0u, (uint32_t)(fresh + 1)) == 0u;
+      if (!placed && pinned) atom::add(empty_pinned_word(tree), 1);
+    }
+    atom::exch(a, fresh < 0 || placed ? 0u : (uint32_t)(fresh + 1));
+    if (last) gather_after_last(tree);
+  }
+
+  // Publish the result of a slot's block search (the slot is held busy):
+  // the block found, else a block fetched ahead for a slot of the class (one
+  // may have arrived during the search), else 0 (empty). Returns whether the
+  // slot got a block.
+  __device__ bool publish_searched(uint32_t tree, uint32_t index, int64_t fresh) {
+    const uint32_t entry = fresh >= 0 ? (uint32_t)(fresh + 1) : take_any_ahead(tree);
+    atom::exch(slot_ptr(tree, index), entry);
+    if (entry == 0u && index < pinned_slots_of(tree)) atom::add(empty_pinned_word(tree), 1);
+    return entry != 0u;
+  }
+
   // Swap a spent pinned block for a fresh one (reference :1083-1138). The
   // slot is marked busy while the new block is found, so that other groups
   // wait for it instead of racing to refill it; it is left empty when memory
-  // runs out.
-  __device__ __noinline__ bool replace_block(uint32_t tree, uint32_t index, uint64_t block, uint64_t tid) {
+  // runs out. A block fetched ahead for the slot is published without a
+  // search, and the slot's next block is fetched. With kFetchOnly in
+  // `tree_flags` it only fetches the slot's next block (the claim that
+  // crossed slice kAheadAt, under load).
+  __device__ __noinline__ bool replace_block(uint32_t tree_flags, uint32_t index, uint64_t block, uint64_t tid) {
+    const uint32_t tree = tree_flags & ~kFetchOnly;
+    if (tree_flags & kFetchOnly) {
+      fetch_ahead(tree, index, tid);
+      return true;
+    }
     uint32_t* s = slot_ptr(tree, index);
     if (atom::cas(s, (uint32_t)(block + 1), kSlotBusy) != (uint32_t)(block + 1)) return true;
     // The slot may hold a later incarnation of the same block: between the
@@ -326,23 +425,30 @@ struct Gallatin {
     // recycled, granted again and published in this very slot. Only a block
     // that is spent (4096 slices handed out) or serves another class is
     // replaced; a live one is put back. With one lane the block is always
-    // the spent one.
+    // the spent one. The ahead word is read in the same round trip.
     const uint64_t now = atom::load(&table.blocks[block].malloc_counter);
+    const uint32_t ahead = atom::load(ahead_ptr(tree, index));
     if (counter_class(now) == tree && counter_count(now) < kSlicesPerBlock) {
       atom::exch(s, (uint32_t)(block + 1));
       return true;
     }
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatReplace);
+    const uint32_t taken = take_ahead(tree, index, ahead);
+    if (taken != 0u) {
+      atom::exch(s, taken);
+      count_stat(control, kStatReplaceTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
+      fetch_ahead(tree, index, tid);  // the slot's next block
+      return true;
+    }
     atom::order();
     bool last = false;
     const int64_t fresh = request_new_block_from_tree(tree, tid, last);
     atom::order();  // the block's counter reset precedes its publication
-    atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
-    if (fresh < 0 && index < pinned_slots_of(tree)) atom::add(empty_pinned_word(tree), 1);
+    const bool got = publish_searched(tree, index, fresh);
     count_stat(control, kStatReplaceTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
     if (last) gather_after_last(tree);
-    return fresh >= 0;
+    return got;
   }
 
   // refill an empty slot (left so by an out-of-memory replace); false when
@@ -357,14 +463,18 @@ struct Gallatin {
     }
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     count_stat(control, kStatRefill);
+    const uint32_t taken = take_ahead(tree, index, atom::load(ahead_ptr(tree, index)));
+    if (taken != 0u) {
+      atom::exch(s, taken);
+      return true;
+    }
     bool last = false;
     const int64_t fresh = request_new_block_from_tree(tree, tid, last);
     atom::order();
-    atom::exch(s, fresh < 0 ? 0u : (uint32_t)(fresh + 1));
-    if (fresh < 0 && pinned) atom::add(empty_pinned_word(tree), 1);
+    const bool got = publish_searched(tree, index, fresh);
     count_stat(control, kStatRefillTime, (uint32_t)((__builtin_amdgcn_s_memrealtime() - t0) >> 4));
     if (last) gather_after_last(tree);
-    return fresh >= 0;
+    return got;
   }
 
   // nothing a block request seeded with `tid` could find: its hashed search
@@ -440,8 +550,10 @@ struct Gallatin {
   // the class. `allocs` is the number of allocations a lane's `count` slices
   // stand for: 1 for malloc, the run length for a bulk claim. `spread`
   // false keeps every group on the reference's pinned-slot walk (no start at
-  // an overflow slot), for deterministic multi-lane replays.
-  template <bool spread = true>
+  // an overflow slot), for deterministic multi-lane replays. `ahead` lets a
+  // group of a multi-lane claim fetch its slot's next block ahead of time
+  // (fetch_ahead); the deterministic replays of the spread form run without.
+  template <bool spread = true, bool ahead = spread>
   __device__ __forceinline__ void claim_pass(uint64_t looking, uint32_t tree, uint32_t count, uint32_t allocs,
                                              uint64_t tid, uint32_t hint, uint32_t subgroup, uint32_t attempt,
                                              uint64_t& result, bool& done, bool& oom, bool& waited) {
@@ -578,6 +690,7 @@ struct Gallatin {
     const uint32_t c0 = counter_count(merged);
     const uint64_t allocation = (uint64_t)c0 + prefix;
     const bool should_replace = has_block && allocation <= 4095 && allocation + count > 4095;
+    const bool should_fetch = spread && ahead && multi && has_block && allocation <= kAheadAt && allocation + count > kAheadAt;
     const bool valid = has_block && allocation + count <= kSlicesPerBlock;
     const uint64_t valid_lanes = wave::ballot(valid) & group;
     // Slices of the claimed range that lie inside the block but belong to no
@@ -616,7 +729,10 @@ struct Gallatin {
 
     bool refilled = true;
     const bool empty = entry == 0u;
-    const uint64_t rare = wave::ballot(is_leader && (empty || replace || completed || give_back > 0)) & looking;
+    // the group whose range crosses slice kAheadAt fetches the slot's next block
+    const bool fetch = ((wave::ballot(should_fetch) & group) != 0) && !replace && !stale;
+    const uint64_t rare =
+        wave::ballot(is_leader && (empty || replace || completed || give_back > 0 || fetch)) & looking;
     for (uint64_t todo = rare; todo; todo &= todo - 1) {
       if ((int)me == wave::first_lane(todo)) {
         const uint64_t lead_tid = tid;
@@ -624,7 +740,7 @@ struct Gallatin {
           // `dry` / `spare`: this pass's first look already answered the test
           refilled = !dry && (spare || !class_out_of_memory(tree, lead_tid)) && refill_slot(tree, index, lead_tid);
         } else {
-          if (replace) replace_block(tree, index, block, lead_tid);
+          if (replace || fetch) replace_block(tree | (replace ? 0u : kFetchOnly), index, block, lead_tid);
           if (completed) recycle_block(block);  // the write-off finished the block
           if (give_back > 0) free_slices(block, give_back);
         }
@@ -651,7 +767,7 @@ struct Gallatin {
   // structurizer runs a loop's exit block once, after the loop, for all lanes
   // that left, so per-group work must not sit in one. `oom` is set when the
   // failure is a confirmed out-of-memory, which a further call would repeat.
-  template <bool spread = true>
+  template <bool spread = true, bool ahead = spread>
   __device__ __forceinline__ uint64_t malloc_slice_allocation(uint32_t tree, uint32_t count, uint32_t allocs,
                                                               uint64_t tid, uint32_t hint, uint32_t subgroup,
                                                               bool& oom) {
@@ -666,7 +782,7 @@ struct Gallatin {
     for (int pass = 0; pass < kBusyPasses && looking; pass++) {
       bool waited = false;
       if ((looking >> wave::lane_id()) & 1ull)
-        claim_pass<spread>(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)pass, result, done, oom,
+        claim_pass<spread, ahead>(looking, tree, count, allocs, tid, hint, subgroup, (uint32_t)pass, result, done, oom,
                            waited);
       if (!done && !waited) tries++;
       looking &= ~wave::ballot(done || tries >= (uint32_t)kSliceAttempts);
@@ -786,8 +902,9 @@ struct Gallatin {
   // malloc with the slot selector and hash id given explicitly; malloc()
   // passes the hardware CU id and the global thread id. `subgroup` (0 for
   // malloc) splits the lanes of one class into separate claim groups: the
-  // coalescing width of the batched kernels. `spread` as in claim_pass.
-  template <bool spread = true>
+  // coalescing width of the batched kernels. `spread` and `ahead` as in
+  // claim_pass.
+  template <bool spread = true, bool ahead = spread>
   __device__ __forceinline__ void* malloc_at(uint64_t size, uint64_t tid, uint32_t hint, uint32_t subgroup = 0) {
     uint32_t tree = get_tree_id_from_size(size);
     uint32_t count = 1;
@@ -800,7 +917,7 @@ struct Gallatin {
     uint64_t offset = utils::kFail;
     bool oom = false;
     for (int attempt = 0; attempt < kMallocLoopAttempts && offset == utils::kFail && !oom; attempt++)
-      offset = malloc_slice_allocation<spread>(tree, count, 1u, tid, hint, subgroup, oom);
+      offset = malloc_slice_allocation<spread, ahead>(tree, count, 1u, tid, hint, subgroup, oom);
     if (offset == utils::kFail) return nullptr;
     return offset_to_allocation(offset, tree);
   }

@@ -90,7 +90,9 @@ enum control_word : uint32_t {
   kStatRecycle = 6,       //   blocks recycled after all slices came back
   kStatReplaceTime = 7,   //   wall-clock ticks / 16 spent in replacements
   kStatRefillTime = 8,    //   wall-clock ticks / 16 spent in refills
-  kStatSites = 9,
+  kStatAhead = 9,         //   blocks fetched ahead for a slot
+  kStatAheadTaken = 10,   //   replacements and refills served by a block fetched ahead
+  kStatSites = 11,
   kCtlWords = 32
 };
 // Each control word has a 128-byte line of its own: the class locks, the

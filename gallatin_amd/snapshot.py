@@ -34,6 +34,7 @@ def parse(buf, num_trees=None):
     queues = take(nblocks * 4, np.uint32)
     segs = take(nseg * 16, np.uint32).reshape(nseg, 4)
     slot_arr = take(slots * 4, np.uint32)
+    ahead_arr = take(slots * 4, np.uint32)
     return {
         "num_segments": nseg,
         "blocks_per_segment": bps,
@@ -51,6 +52,7 @@ def parse(buf, num_trees=None):
         "queue_free": segs[:, 2],
         "queue_final": segs[:, 3],
         "slots": slot_arr,
+        "ahead": ahead_arr,
     }
 
 

@@ -97,6 +97,9 @@ typedef struct gal_info_t {
  *   u32 queue[num_segments*blocks_per_segment]  (0 empty, else block id + 1)
  *   u32 {active, queue_alloc, queue_free, queue_final} per segment (active as i32)
  *   u32 slots[total_slots], class-major (0 empty, else block id + 1)
+ *   u32 ahead[total_slots], same order: the block fetched ahead for each slot
+ *       (0 none, else block id + 1); always 0 in the oracle, which models no
+ *       fetch ahead (one lane never fetches ahead)
  *   zero-padded to a multiple of 8 bytes
  */
 
@@ -159,7 +162,8 @@ int gal_replay(gal_allocator* h, const gal_op* ops, uint64_t n, uint64_t* result
  * every group; GAL_OP_MALLOC_SPREAD claims are the ones malloc makes under
  * load (when more than one lane claims, a group's first pass starts at slot
  * `slot` % usable of its class, pinned or overflow, and fills it if empty
- * unless the class is out of memory). Either way the results are a function
+ * unless the class is out of memory), without the blocks that malloc fetches
+ * ahead for its slots under load. Either way the results are a function
  * of the op stream alone and the CPU oracle reproduces them bit for bit.
  *   GAL_OP_MALLOC, GAL_OP_MALLOC_SPREAD:
  *                  arg[l] = request size of lane l. The lanes' sizes are all

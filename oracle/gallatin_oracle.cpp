@@ -1169,7 +1169,7 @@ struct Alloc {
     uint64_t bytes = words * 8;
     bytes += ((table.num_segments * 2 + 7) / 8) * 8;
     uint64_t nblocks = table.num_segments * table.bps;
-    bytes += nblocks * 8 + nblocks * 4 + table.num_segments * 16 + total_slots() * 4;
+    bytes += nblocks * 8 + nblocks * 4 + table.num_segments * 16 + total_slots() * 8;
     return ((bytes + 7) / 8) * 8;
   }
 
@@ -1204,6 +1204,10 @@ struct Alloc {
       put(slots[c].data(), slots[c].size() * 4);
       if (!overflow[c].empty()) put(overflow[c].data(), overflow[c].size() * 4);
     }
+    // the GPU's ahead word per slot: the GPU fetches blocks ahead only for
+    // claims under load, which this model does not replay, so every word is 0
+    std::memset(p, 0, total_slots() * 4);
+    p += total_slots() * 4;
     size_t used = (size_t)(p - out);
     size_t total = snapshot_bytes();
     if (total > used) std::memset(p, 0, total - used);

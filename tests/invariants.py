@@ -111,8 +111,12 @@ def check_quiescent(image, seg_bytes, min_size):
                               f"({[hex(lower_vals[w]) for w in lost[:4]]}) but their summary bits in layer "
                               f"{level} are clear (layer {level}: {[hex(int(x)) for x in np.asarray(upper)[:8]]})")
 
+    # blocks held by slots: each slot's entry and the block fetched ahead for it
     slots = s["slots"].astype(np.int64)
-    pinned = slots[slots > 0] - 1
+    ahead = s["ahead"].astype(np.int64)
+    assert not np.any(slots == 0xFFFFFFFF), "a slot is still busy"
+    assert not np.any(ahead == 0xFFFFFFFF), "a fetch ahead is unfinished"
+    pinned = np.concatenate([slots[slots > 0], ahead[ahead > 0]]) - 1
     assert len(np.unique(pinned)) == len(pinned), "a block is pinned twice"
     pinned_per_seg = np.bincount(pinned // bps, minlength=nseg)
 
@@ -146,4 +150,11 @@ def check_quiescent(image, seg_bytes, min_size):
     assert np.all(fc[is_pinned] == count[is_pinned]), "a pinned block has slices out"
     seg_of = pinned // bps
     assert np.all((mc[pinned] >> 20) == chunk[seg_of]), "pinned block of the wrong class"
+    base = 0
+    for c in range(T):
+        region = ahead[base:base + max(pinned_slots_of(c), 64)]
+        base += max(pinned_slots_of(c), 64)
+        fetched = region[region > 0] - 1
+        assert np.all(count[fetched] == 0), f"a block fetched ahead for class {c} has slices claimed"
+        assert np.all((mc[fetched] >> 20) == c), f"a block of another class fetched ahead for class {c}"
     return s
