@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "a5aef2ae49eac96c",
-    "commit": "ddab923",
-    "traffic": {"mixed_malloc_kernel": 196045548, "mixed_free_kernel": 197670632},
-    "atomic_requests": {"mixed_malloc_kernel": 272433, "mixed_free_kernel": 488515},
+    "sources_sha": "600382510a8c30c0",
+    "commit": "a0fc5ac",
+    "traffic": {"mixed_malloc_kernel": 209778316, "mixed_free_kernel": 206685143},
+    "atomic_requests": {"mixed_malloc_kernel": 334721, "mixed_free_kernel": 587418},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
@@ -402,6 +402,8 @@ def main():
         log(f"[bench] warmup {w}: {r}")
 
     results = []
+    # the latency samples of every timed step, not only the last one's
+    g.latency_keep(args.steps)
     t0 = time.perf_counter()
     for k in range(args.steps):
         # the same launches in one call: every round's malloc and free kernel

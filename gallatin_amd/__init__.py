@@ -315,8 +315,14 @@ class Gallatin:
         )
         return r.as_dict()
 
+    def latency_keep(self, runs):
+        """Keep the latency samples of the next `runs` run_mixed calls (of one
+        shape) for latency_ns, instead of the last run's; 0 restores that."""
+        _capi.check("gal_latency_keep", _wl.gal_latency_keep(self._workload(), runs))
+
     def latency_ns(self):
-        """Per-wave malloc latencies (ns) of the last mixed run."""
+        """Per-wave malloc latencies (ns) of the last mixed run, or of the
+        runs kept since latency_keep."""
         n, tick = ctypes.c_uint64(), ctypes.c_double()
         _capi.check("gal_latency_samples", _wl.gal_latency_samples(self._workload(), None, 0, ctypes.byref(n), ctypes.byref(tick)))
         buf = np.zeros(max(1, n.value), dtype=np.uint32)

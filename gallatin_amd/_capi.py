@@ -240,6 +240,7 @@ _WL_SIGNATURES = {
         ctypes.c_int,
         [vp, ctypes.POINTER(u32), u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)],
     ),
+    "gal_latency_keep": (ctypes.c_int, [vp, u32]),
     "gal_slab_create": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(vp)]),
     "gal_slab_insert": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(GalSlabResult)]),
     "gal_slab_query": (ctypes.c_int, [vp, u64, u64, ctypes.POINTER(GalSlabResult)]),

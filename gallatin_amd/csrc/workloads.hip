@@ -77,10 +77,15 @@ struct gal_workload {
   uint32_t rounds = 0;
   uint32_t rounds_done = 0;
   uint64_t mixed_seed = 0;
-  // per-wave malloc latency samples (device wall-clock ticks) of the last run
+  // per-wave malloc latency samples (device wall-clock ticks) of the last
+  // run, or of the runs kept since gal_latency_keep (lat_keep of them, each
+  // lat_per_run samples long, lat_kept recorded so far)
   uint32_t* d_lat = nullptr;
   uint64_t lat_cap = 0;
   uint64_t lat_count = 0;
+  uint32_t lat_keep = 0;
+  uint32_t lat_kept = 0;
+  uint64_t lat_per_run = 0;
   // batches, serving loops and slab tables made on this workload and not
   // yet destroyed: they use its counters and events, so it outlives them
   int children = 0;
@@ -946,7 +951,9 @@ int gal_mixed_reset(gal_workload* h, uint64_t threads_per_round, uint32_t rounds
         return GAL_ERR_OOM;
     h->region_cap = regions;
   }
-  const uint64_t lat_need = (need + 63) / 64 + 1;
+  const uint64_t per_run = (need + 63) / 64;
+  if (h->lat_keep && h->lat_kept > 0 && per_run != h->lat_per_run) return GAL_ERR_INVALID;  // kept runs differ in shape
+  const uint64_t lat_need = per_run * (h->lat_keep ? h->lat_keep : 1) + 1;
   if (lat_need > h->lat_cap) {
     if (h->d_lat) (void)hipFree(h->d_lat);
     h->d_lat = nullptr;
@@ -954,7 +961,8 @@ int gal_mixed_reset(gal_workload* h, uint64_t threads_per_round, uint32_t rounds
     if (hipMalloc(&h->d_lat, lat_need * 4) != hipSuccess) return GAL_ERR_OOM;
     h->lat_cap = lat_need;
   }
-  h->lat_count = 0;
+  if (!h->lat_keep) h->lat_count = 0;
+  h->lat_per_run = per_run;
   h->threads_per_round = threads_per_round;
   h->rounds = rounds;
   h->rounds_done = 0;
@@ -969,7 +977,7 @@ int gal_mixed_round(gal_workload* h, uint32_t round, uint32_t verify, gal_run_re
   int rc = timed_run(h, out, [&](launch_timer& t) {
     if (!t.begin()) return GAL_ERR_INVALID;
     h->ops->launch_mixed_malloc(h, (uint64_t)round * h->threads_per_round, h->threads_per_round, round, seed, verify,
-                                h->d_lat);
+                                h->lat_keep ? nullptr : h->d_lat);
     t.end(0);
     if (!t.begin()) return GAL_ERR_INVALID;
     h->ops->launch_mixed_free(h, h->threads_per_round, (uint64_t)round * h->threads_per_round, round, seed, verify,
@@ -979,7 +987,7 @@ int gal_mixed_round(gal_workload* h, uint32_t round, uint32_t verify, gal_run_re
   });
   if (rc == GAL_OK) {
     h->rounds_done++;
-    h->lat_count = ((uint64_t)h->rounds_done * h->threads_per_round + 63) / 64;
+    if (!h->lat_keep) h->lat_count = ((uint64_t)h->rounds_done * h->threads_per_round + 63) / 64;
   }
   return rc;
 }
@@ -1032,10 +1040,14 @@ int gal_run_mixed(gal_workload* h, uint64_t threads_per_round, uint32_t rounds, 
   if (rc != GAL_OK) return rc;
   gal_run_result total;
   std::memset(&total, 0, sizeof(total));
+  // samples at the start of the buffer, or after the runs kept so far (none
+  // once gal_latency_keep's count is reached)
+  uint32_t* lat = h->d_lat;
+  if (h->lat_keep) lat = h->lat_kept < h->lat_keep ? h->d_lat + (uint64_t)h->lat_kept * h->lat_per_run : nullptr;
   rc = timed_run(h, &total, [&](launch_timer& t) {
     for (uint32_t r = 0; r < rounds; r++) {
       if (!t.begin()) return GAL_ERR_INVALID;
-      h->ops->launch_mixed_malloc(h, (uint64_t)r * threads_per_round, threads_per_round, r, seed, verify, h->d_lat);
+      h->ops->launch_mixed_malloc(h, (uint64_t)r * threads_per_round, threads_per_round, r, seed, verify, lat);
       t.end(0);
       if (!t.begin()) return GAL_ERR_INVALID;
       h->ops->launch_mixed_free(h, threads_per_round, (uint64_t)r * threads_per_round, r, seed, verify, false);
@@ -1048,10 +1060,24 @@ int gal_run_mixed(gal_workload* h, uint64_t threads_per_round, uint32_t rounds, 
   });
   if (rc == GAL_OK) {
     h->rounds_done = 0;  // drained
-    h->lat_count = ((uint64_t)rounds * threads_per_round + 63) / 64;
+    if (!h->lat_keep) {
+      h->lat_count = h->lat_per_run;
+    } else if (lat) {
+      h->lat_kept++;
+      h->lat_count = (uint64_t)h->lat_kept * h->lat_per_run;
+    }
   }
   if (out) *out = total;
   return rc;
+}
+
+int gal_latency_keep(gal_workload* h, uint32_t runs) {
+  if (!h) return GAL_ERR_INVALID;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return GAL_ERR_HIP;
+  h->lat_keep = runs;
+  h->lat_kept = 0;
+  h->lat_count = 0;
+  return GAL_OK;
 }
 
 int gal_latency_samples(gal_workload* h, uint32_t* ticks, uint64_t cap, uint64_t* count, double* tick_ns) {

@@ -82,10 +82,17 @@ int gal_mixed_live(gal_workload* w, uint64_t* offsets, uint64_t* sizes, uint64_t
 int gal_run_mixed(gal_workload* w, uint64_t threads_per_round, uint32_t rounds, uint64_t seed, uint32_t verify,
                   gal_run_result* out);
 
-/* Malloc latency samples of the last mixed run: one per wavefront (its lane
- * 0), the wave's time inside malloc in device wall-clock ticks; tick_ns is
- * the tick length. ticks == NULL just reports the count. */
+/* Malloc latency samples of the last mixed run (or of the runs kept, see
+ * gal_latency_keep): one per wavefront (its lane 0) and round, the wave's
+ * time inside malloc in device wall-clock ticks; tick_ns is the tick
+ * length. ticks == NULL just reports the count. */
 int gal_latency_samples(gal_workload* w, uint32_t* ticks, uint64_t cap, uint64_t* count, double* tick_ns);
+/* Keep the samples of the next `runs` gal_run_mixed calls one after another
+ * (runs of one shape; later runs record none) instead of the last run's:
+ * gal_latency_samples then returns them all, so percentiles cover several
+ * runs. gal_mixed_round records no samples meanwhile. runs == 0 restores
+ * the default. Clears the samples held. */
+int gal_latency_keep(gal_workload* w, uint32_t runs);
 
 /* ---- workloads of the global allocator ----
  * They run on the global allocators made by gal_init_global_allocator(_host)

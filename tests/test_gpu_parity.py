@@ -232,6 +232,20 @@ def test_mixed_full_workload_repeatable():
             assert res["tag_errors"] == 0 and res["alloc_errors"] == 0, (res, g.error_sites())
             assert res["frees"] == res["mallocs"]
             invariants.check_quiescent(g.snapshot(), 16 * MiB, 16)
+        # latency samples: one per wave and round of the last run, or of the
+        # runs kept (gal_latency_keep; runs past the count record none)
+        per_run = rounds * threads // 64
+        assert len(g.latency_ns()) == per_run
+        g.latency_keep(2)
+        assert len(g.latency_ns()) == 0
+        for step in range(3):
+            res = g.run_mixed(threads, rounds, 200 + step)
+            assert res["tag_errors"] == 0 and res["alloc_errors"] == 0
+            lat = g.latency_ns()
+            assert len(lat) == per_run * min(step + 1, 2) and np.all(lat > 0)
+        g.latency_keep(0)
+        g.run_mixed(threads, rounds, 300)
+        assert len(g.latency_ns()) == per_run
     finally:
         g.free_on_device()
 
