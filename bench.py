@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "600382510a8c30c0",
-    "commit": "a0fc5ac",
-    "traffic": {"mixed_malloc_kernel": 209778316, "mixed_free_kernel": 206685143},
-    "atomic_requests": {"mixed_malloc_kernel": 334721, "mixed_free_kernel": 587418},
+    "sources_sha": "aa6cba5bfcadd237",
+    "commit": "25a3af9",
+    "traffic": {"mixed_malloc_kernel": 207809702, "mixed_free_kernel": 205596480},
+    "atomic_requests": {"mixed_malloc_kernel": 336146, "mixed_free_kernel": 591535},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
