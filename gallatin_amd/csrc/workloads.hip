@@ -62,6 +62,7 @@ struct gal_workload {
   uint32_t* control = nullptr;   // device control words of the allocator
   hipEvent_t ev[2 * kEventPairs] = {};
   run_counters* d_counters = nullptr;
+  run_counters* h_counters = nullptr;  // pinned host copy, written on the stream
   double tick_ns = 10.0;  // device wall-clock tick
   // mixed workload state: the round's malloc results (one pointer per
   // thread), and the survivors of the rounds so far in per-wave regions of
@@ -721,18 +722,29 @@ const wl_ops* find_impl(uint64_t seg, uint64_t min, uint64_t max) {
 }
 
 
-// events around launches on the handle's stream
+// Events around launches on the handle's stream. A launch is timed from
+// the event before it to the event after it; with `chain` set, launches
+// that follow each other with nothing between them on the stream share
+// their boundary event (one event a launch instead of two).
 struct launch_timer {
   gal_workload* h;
-  int used = 0;
-  std::vector<int> kinds;  // 0 malloc kernel, 1 free-only kernel
+  bool chain = false;
+  int used = 0;              // launches timed
+  int events = 0;            // events recorded
+  std::vector<int> kinds;    // 0 malloc kernel, 1 free-only kernel
+  std::vector<int> starts;   // event before each launch (the one after is next)
   explicit launch_timer(gal_workload* hh) : h(hh) {}
   bool begin() {
-    if (used >= kEventPairs) return false;
-    return hipEventRecord(h->ev[2 * used], h->stream) == hipSuccess;
+    if (events + 2 > 2 * kEventPairs) return false;
+    if (chain && events > 0) {
+      starts.push_back(events - 1);
+      return true;
+    }
+    starts.push_back(events);
+    return hipEventRecord(h->ev[events++], h->stream) == hipSuccess;
   }
   void end(int kind) {
-    (void)hipEventRecord(h->ev[2 * used + 1], h->stream);
+    (void)hipEventRecord(h->ev[events++], h->stream);
     kinds.push_back(kind);
     used++;
   }
@@ -741,7 +753,7 @@ struct launch_timer {
     if (hipGetLastError() != hipSuccess) return GAL_ERR_HIP;
     for (int i = 0; i < used; i++) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]) != hipSuccess) return GAL_ERR_HIP;
+      if (hipEventElapsedTime(&ms, h->ev[starts[i]], h->ev[starts[i] + 1]) != hipSuccess) return GAL_ERR_HIP;
       if (kinds[i] == 0)
         out->malloc_kernel_ms += ms;
       else
@@ -749,7 +761,7 @@ struct launch_timer {
     }
     if (used > 0) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, h->ev[0], h->ev[2 * used - 1]) != hipSuccess) return GAL_ERR_HIP;
+      if (hipEventElapsedTime(&ms, h->ev[0], h->ev[events - 1]) != hipSuccess) return GAL_ERR_HIP;
       out->total_ms += ms;
     }
     out->launches += (uint64_t)used;
@@ -757,29 +769,47 @@ struct launch_timer {
   }
 };
 
+// Counters zeroed and the allocator's error count taken before a run, the
+// error count taken again after it: one small launch each, on the stream.
+// The error word is read with a read-modify-write that changes nothing, so
+// it counts every error an atomic recorded before.
+__global__ void run_begin_kernel(run_counters* c, uint32_t* errors) {
+  unsigned long long* w = (unsigned long long*)c->s;
+  const uint32_t n = (uint32_t)(sizeof(c->s) / sizeof(unsigned long long));
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0ull;
+  if (threadIdx.x == 0) {
+    c->errors_before = atom::read_current(errors);
+    c->errors_after = 0u;
+  }
+}
+__global__ void run_end_kernel(run_counters* c, uint32_t* errors) {
+  if (threadIdx.x == 0) c->errors_after = atom::read_current(errors);
+}
+
 // run `body` (which launches through the timer) with zeroed counters; fill out
 template <class F>
 int timed_run(gal_workload* h, gal_run_result* out, F&& body) {
   gal_run_result r;
   std::memset(&r, 0, sizeof(r));
-  // the error word is copied before and after the kernels on the same stream,
-  // so that one read after the run returns both with the counters
+  // the error word is taken before and after the kernels on the same stream,
+  // and the counters come back to pinned host memory on the stream, so that
+  // the one synchronisation of the run returns them all
   uint32_t* errors = ga::ctl_word(h->control, ga::kCtlErrors);
-  if (hipMemsetAsync(h->d_counters, 0, sizeof(run_counters), h->stream) != hipSuccess ||
-      hipMemcpyAsync(&h->d_counters->errors_before, errors, 4, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
-    return GAL_ERR_HIP;
+  run_begin_kernel<<<1, 256, 0, h->stream>>>(h->d_counters, errors);
+  if (hipGetLastError() != hipSuccess) return GAL_ERR_HIP;
   launch_timer t(h);
   int rc = body(t);
   if (rc != GAL_OK) {
     (void)hipStreamSynchronize(h->stream);
     return rc;
   }
-  if (hipMemcpyAsync(&h->d_counters->errors_after, errors, 4, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+  run_end_kernel<<<1, 64, 0, h->stream>>>(h->d_counters, errors);
+  if (hipMemcpyAsync(h->h_counters, h->d_counters, sizeof(run_counters), hipMemcpyDeviceToHost, h->stream) !=
+      hipSuccess)
     return GAL_ERR_HIP;
   rc = t.finish(&r);
   if (rc != GAL_OK) return rc;
-  run_counters c;
-  if (hipMemcpy(&c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return GAL_ERR_HIP;
+  const run_counters& c = *h->h_counters;
   const uint64_t err0 = c.errors_before, err1 = c.errors_after;
   for (const counter_stripe& k : c.s) {
     r.mallocs += k.mallocs;
@@ -860,6 +890,8 @@ int make_workload(gal_allocator* h, bool hold, gal_workload** out) {
   for (auto& e : w->ev)
     if (rc == GAL_OK && hipEventCreate(&e) != hipSuccess) rc = GAL_ERR_HIP;
   if (rc == GAL_OK && hipMalloc(&w->d_counters, sizeof(run_counters)) != hipSuccess) rc = GAL_ERR_OOM;
+  if (rc == GAL_OK && hipHostMalloc((void**)&w->h_counters, sizeof(run_counters), hipHostMallocDefault) != hipSuccess)
+    rc = GAL_ERR_OOM;
   int khz = 0, device = 0;
   if (hipGetDevice(&device) == hipSuccess &&
       hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
@@ -896,6 +928,7 @@ int gal_workload_destroy(gal_workload* w) {
   }
   if (w->d_lat) (void)hipFree(w->d_lat);
   if (w->d_counters) (void)hipFree(w->d_counters);
+  if (w->h_counters) (void)hipHostFree(w->h_counters);
   for (auto& e : w->ev)
     if (e) (void)hipEventDestroy(e);
   if (w->holds) (void)gal_release(w->owner);
@@ -1045,6 +1078,7 @@ int gal_run_mixed(gal_workload* h, uint64_t threads_per_round, uint32_t rounds, 
   uint32_t* lat = h->d_lat;
   if (h->lat_keep) lat = h->lat_kept < h->lat_keep ? h->d_lat + (uint64_t)h->lat_kept * h->lat_per_run : nullptr;
   rc = timed_run(h, &total, [&](launch_timer& t) {
+    t.chain = true;  // the launches follow each other with nothing between
     for (uint32_t r = 0; r < rounds; r++) {
       if (!t.begin()) return GAL_ERR_INVALID;
       h->ops->launch_mixed_malloc(h, (uint64_t)r * threads_per_round, threads_per_round, r, seed, verify, lat);
