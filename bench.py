@@ -45,10 +45,10 @@ BYTES_FREE_SIDE = CHUNK_ID + 2 * FREE_COUNTER + TAG  # 20 B per free
 #     the lanes of one instruction within one 64-B span),
 #     profiles/mixed_slices_8GiB_tcc_atomics.md.
 MEASURED = {
-    "sources_sha": "aa6cba5bfcadd237",
-    "commit": "25a3af9",
-    "traffic": {"mixed_malloc_kernel": 207809702, "mixed_free_kernel": 205596480},
-    "atomic_requests": {"mixed_malloc_kernel": 336146, "mixed_free_kernel": 591535},
+    "sources_sha": "4ea174eecb2597b5",
+    "commit": "1907a23",
+    "traffic": {"mixed_malloc_kernel": 207444995, "mixed_free_kernel": 205178647},
+    "atomic_requests": {"mixed_malloc_kernel": 335218, "mixed_free_kernel": 589108},
 }
 # the sources the mixed workload's kernels are compiled from: their kernels
 # (workloads.hip) and the device allocator (headers)
